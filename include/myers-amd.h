@@ -1,0 +1,157 @@
+/*
+ * myers-amd.h -- extensions of the candidate-verification library
+ * (libmyers_amd.so, MI355X / gfx950) beyond the reference API of
+ * myers-interface.h.  Plain C99; no HIP or C++ types in any signature
+ * (device pointers and streams are passed as void *).
+ *
+ * Reference counterparts are cited per entry point; entry points marked
+ * "extension" have none in the reference.
+ */
+#ifndef MYERS_AMD_H
+#define MYERS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "myers-interface.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes.  0..5 keep the values of the reference's myersError_t
+ * (src-api/myers-common.h:13-24); the rest are extensions. */
+typedef enum {
+    MYERS_SUCCESS                = 0,
+    MYERS_E_OPENING_FILE         = 1,
+    MYERS_E_READING_FILE         = 2,
+    MYERS_E_INSUFFICIENT_MEM_GPU = 3,
+    MYERS_E_ALLOCATE_MEM         = 4,
+    MYERS_E_INCOMPATIBLE_GPU     = 5,
+    MYERS_E_INVALID_ARGUMENT     = 16,  /* NULL handle, zero buffers, bad flag */
+    MYERS_E_CAPACITY             = 17,  /* batch larger than the buffer's capacity */
+    MYERS_E_HIP                  = 18,  /* a HIP runtime call failed */
+    MYERS_E_QUERY_TOO_LONG       = 19,  /* a query exceeds the longest supported length;
+                                           its candidates got the skip sentinel */
+    MYERS_E_NO_DEVICE            = 20,  /* no HIP device visible */
+    MYERS_E_REFERENCE_TOO_LARGE  = 21   /* reference words do not fit 32-bit indexing */
+} myersError_t;
+
+/* Window semantics (flags of myersSendEx / myersVerifyDevice). */
+#define MYERS_WINDOW_ALIGNED   0u  /* API kernel: window starts at the 8-base word holding the
+                                      position, 8*(n/8 + (n%8 ? 2 : 1)) columns, columns counted
+                                      from that word (src-api/coreMyersGPU-CC35.cu:132,144,164) */
+#define MYERS_WINDOW_EXACT     1u  /* exact start: n columns from the position
+                                      (src/myers-4bits.c:100-147) */
+/* Engine selection (extension). */
+#define MYERS_ENGINE_AUTO       0u
+#define MYERS_ENGINE_PLAIN      (1u << 4)  /* one launch per length bin */
+#define MYERS_ENGINE_PERSISTENT (2u << 4)  /* one persistent launch over a length-binned queue */
+#define MYERS_ENGINE_MASK       (3u << 4)
+
+/* Longest query supported, in bases (extension: the reference stops at 4096,
+ * src-api/myersGPU.c:366,406-410). */
+#define MYERS_MAX_QUERY_SIZE   24576u
+
+typedef struct {
+    uint32_t structSize;        /* sizeof(myersConfig_t) */
+    int32_t  device;            /* HIP device index (default 0) */
+    int32_t  exitOnError;       /* 1: print and exit on errors in the void API calls,
+                                   as the reference does (src-api/myersGPU.c:24-47) */
+    uint32_t reserved;
+    uint64_t bufferBytes;       /* device bytes per buffer used to size capacities;
+                                   0 = min(95% of free HBM / numBuffers, 2 GiB) */
+} myersConfig_t;
+
+/* Timing and counters of a buffer's last batch (extension). */
+typedef struct {
+    uint32_t structSize;
+    uint32_t launches;          /* verification kernel launches in the last batch */
+    uint32_t variantsUsed;      /* distinct kernel variants in the last batch */
+    uint32_t binned;            /* 1 if the batch went through device-side binning */
+    double   kernelMs;          /* HIP-event time of the verification kernels, last batch */
+    double   batchMs;           /* HIP-event time from first copy to last copy */
+    uint64_t cells;             /* sum of m * n over verified candidates (nominal window) */
+    uint64_t candidates;
+    /* accumulated over the buffer's life (or since myersResetStats) */
+    double   kernelMsTotal;
+    uint64_t kernelLaunchesTotal;
+    uint64_t batchesTotal;
+} myersStats_t;
+
+void myersDefaultConfig(myersConfig_t *cfg);
+
+/* initMyers with a configuration (extension of src-api/myersGPU.c:284-344). */
+int myersInitEx(void ***myersBuffer, uint32_t numBuffers, const char *referenceFileName,
+                int32_t averageQuerySize, int32_t candidatesPerQuery, const myersConfig_t *cfg);
+
+/* As myersInitEx, with the packed reference taken from host memory:
+ * numWords u32 words of 8 bases each (base i of a word in bits [4i, 4i+3]),
+ * refSize bases (extension; 64-bit size lifts the 4.29 Gbase header limit
+ * of src/genReference-4bits.c:134-135). */
+int myersInitFromMemory(void ***myersBuffer, uint32_t numBuffers, const uint32_t *refWords,
+                        uint64_t numWords, uint64_t refSize, int32_t averageQuerySize,
+                        int32_t candidatesPerQuery, const myersConfig_t *cfg);
+
+/* sendMyersBuffer returning an error code, with window/engine flags
+ * (src-api/myersGPU.c:466-494). */
+int myersSendEx(void *myersBuffer, float distance, uint32_t numPEQEntries, uint32_t numQueries,
+                uint32_t numCandidates, uint32_t flags);
+
+/* receiveMyersBuffer returning an error code (src-api/myersGPU.c:515-524). */
+int myersReceiveEx(void *myersBuffer);
+
+/* endMyers returning an error code (src-api/myersGPU.c:651-676). */
+int myersEnd(void ***myersBuffer);
+
+const char *myersGetErrorString(int error);     /* src-api/myersGPU.c:31-40 */
+int         myersGetLastError(void *myersBuffer);  /* extension */
+int         myersGetStats(void *myersBuffer, myersStats_t *out);  /* extension */
+void        myersResetStats(void *myersBuffer);                   /* extension */
+uint32_t    myersGetNumBuffers(void *myersBuffer);                /* extension */
+uint64_t    myersGetReferenceSize(void *myersBuffer);             /* extension */
+
+/* ---- Device-resident batches (extension) ------------------------------
+ * For callers whose queries and candidates already sit in HBM: no host
+ * copies; the batch runs on the buffer's stream and uses its binning
+ * workspace (so it needs numCandidates <= getMaxCandidates(buffer)).
+ * h_qinfo (optional, host copy of the qinfo array) lets the library pick
+ * the kernel variants without reading device memory; NULL launches every
+ * variant over the device-side bins.  Candidates naming a query >=
+ * numQueries, or a query whose entries run past numPEQEntries, get the skip
+ * sentinel.  Returns after enqueueing. */
+int myersVerifyDevice(void *myersBuffer, const void *d_peq, uint32_t numPEQEntries, const void *d_qinfo,
+                      uint32_t numQueries, const void *d_candidates, uint32_t numCandidates,
+                      float distance, uint32_t flags, void *d_results, const qryInfo_t *h_qinfo);
+int myersSynchronize(void *myersBuffer);          /* wait for the buffer's stream */
+void *myersGetStream(void *myersBuffer);          /* the buffer's hipStream_t */
+
+/* Device memory helpers so that hosts without a HIP toolchain (ctypes,
+ * JNI, cgo) can stage device-resident batches. */
+int myersDeviceMalloc(void **ptr, size_t bytes);
+int myersDeviceFree(void *ptr);
+int myersMemcpyHtoD(void *dst, const void *src, size_t bytes);
+int myersMemcpyDtoH(void *dst, const void *src, size_t bytes);
+
+/* Kernel variants compiled into the library and the length-bin table
+ * (extension; used by the autotuner).  bin[w] is the variant used for
+ * queries of w = ceil(m/32) words, 0 <= w <= maxWords. */
+typedef struct {
+    uint32_t lanes;          /* lanes of a wave64 per candidate */
+    uint32_t words;          /* u32 words per lane */
+    uint32_t maxQuerySize;   /* 32 * lanes * words */
+    uint32_t candidatesPerWave;
+    uint32_t ldsBytes;
+    int32_t  workgroupsPerCU;
+} myersVariantInfo_t;
+
+uint32_t myersNumVariants(void);
+int      myersGetVariantInfo(void *myersBuffer, uint32_t variant, myersVariantInfo_t *out);
+int      myersGetBinTable(void *myersBuffer, uint8_t *bin, uint32_t maxWords);
+int      myersSetBinTable(void *myersBuffer, const uint8_t *bin, uint32_t maxWords);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MYERS_AMD_H */

@@ -1,0 +1,773 @@
+/*
+ * Host side of the candidate-verification library: the buffer API of
+ * src-api/myers-interface.h, re-designed for one MI355X.
+ *
+ * Plain C over the HIP runtime C API.  Kernels are launched through the
+ * extern "C" shims of myers_kernels.hip (myers_kernels.h).
+ *
+ * Life of a batch (src-api/myersGPU.c:466-524 is the reference flow):
+ *   send:    async H2D of Peq entries, qinfo and candidates on the buffer's
+ *            stream -> kernel launches -> async D2H of the results.  The
+ *            kernels write results in the caller's candidate order, so the
+ *            reference's host reorder (myersGPU.c:351-422) and result
+ *            scatter (myersGPU.c:500-513) do not exist here.
+ *   launch:  the host scans qinfo (O(queries)) for the length bins present.
+ *            One bin -> one launch over the candidates as they are.  More
+ *            bins -> device-side binning (count, scan, scatter) and one
+ *            launch per present bin (plain engine), or a single
+ *            persistent launch (persistent engine, see below).
+ *   receive: hipStreamSynchronize; results are in the pinned results buffer.
+ *
+ * All buffers of a library instance live on one device and share one copy
+ * of the packed reference in HBM (the reference freed it once per buffer,
+ * myersGPU.c:661).
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "myers-interface.h"
+#include "myers-amd.h"
+#include "myers_kernels.h"
+
+#define GUARD_WORDS      64u            /* all-N words after the reference */
+#define ALL_N_WORD       0x44444444u
+#define MAX_QUERY_WORDS  (MYERS_MAX_QUERY_SIZE / 32u)
+#define STAGE_BYTES      (64u << 20)    /* pinned staging for the reference upload */
+#define DEFAULT_BUFFER_CAP (2ull << 30)
+
+typedef struct myers_ctx {
+    int       device;
+    int       num_cus;
+    uint32_t *d_ref;
+    uint64_t  ref_words;         /* stored words */
+    uint64_t  ref_words_alloc;   /* stored + guard */
+    uint64_t  ref_size;          /* bases */
+    uint32_t  num_variants;
+    myers_variant_desc_t variants[MYERS_MAX_VARIANTS];
+    int       occupancy[MYERS_MAX_VARIANTS];
+    uint8_t   bin_of_words[MAX_QUERY_WORDS + 1];
+    uint8_t  *d_bin_of_words;
+} myers_ctx_t;
+
+typedef struct myers_buffer {
+    myers_ctx_t *ctx;
+    uint32_t     num_buffers;
+    uint32_t     id_buffer;
+    uint32_t     max_peq_entries;
+    uint32_t     max_candidates;
+    uint32_t     max_queries;
+    hipStream_t  stream;
+    qryEntry_t  *h_peq;
+    qryEntry_t  *d_peq;
+    qryInfo_t   *h_qinfo;
+    qryInfo_t   *d_qinfo;
+    candInfo_t  *h_cands;
+    candInfo_t  *d_cands;
+    resEntry_t  *h_results;
+    resEntry_t  *d_results;
+    uint32_t    *d_order;
+    uint32_t    *d_work;          /* counts | cursors | ranges(2x) | err */
+    uint32_t    *h_err;           /* pinned copy of the binning error flag */
+    hipEvent_t   ev_batch_start, ev_batch_end, ev_kernel_start, ev_kernel_end;
+    int          exit_on_error;
+    int          last_error;
+    int          pending;         /* a batch was enqueued and not yet received */
+    int          binned;
+    uint32_t     num_candidates;
+    myersStats_t stats;
+} myers_buffer_t;
+
+/* ------------------------------------------------------------------ errors */
+
+const char *myersGetErrorString(int error)
+{
+    switch (error) {
+    case MYERS_SUCCESS:                return "Myers - Success";
+    case MYERS_E_OPENING_FILE:         return "Myers - Error: opening file";
+    case MYERS_E_READING_FILE:         return "Myers - Error: reading file";
+    case MYERS_E_INSUFFICIENT_MEM_GPU: return "Myers - Error: there aren't enough GPU memory space";
+    case MYERS_E_ALLOCATE_MEM:         return "Myers - Error: allocating data";
+    case MYERS_E_INCOMPATIBLE_GPU:     return "Myers - Error: incompatible GPU";
+    case MYERS_E_INVALID_ARGUMENT:     return "Myers - Error: invalid argument";
+    case MYERS_E_CAPACITY:             return "Myers - Error: batch exceeds the buffer capacity";
+    case MYERS_E_HIP:                  return "Myers - Error: HIP runtime call failed";
+    case MYERS_E_QUERY_TOO_LONG:       return "Myers - Error: query longer than the supported maximum";
+    case MYERS_E_NO_DEVICE:            return "Myers - Error: no HIP device";
+    case MYERS_E_REFERENCE_TOO_LARGE:  return "Myers - Error: reference too large";
+    default:                           return "Myers - Unknown error";
+    }
+}
+
+static int hip_fail(hipError_t e, const char *what, const char *file, int line)
+{
+    if (e == hipSuccess) return MYERS_SUCCESS;
+    fprintf(stderr, "Myers - HIP error '%s' in %s (%s:%d)\n", hipGetErrorString(e), what, file, line);
+    return MYERS_E_HIP;
+}
+#define HIP_TRY(call) do { int _rc = hip_fail((call), #call, __FILE__, __LINE__); if (_rc) return _rc; } while (0)
+
+/* The reference prints and exits on any error (src-api/myersGPU.c:24-47);
+ * buffers created by initMyers keep that behaviour for the void calls. */
+static void report(myers_buffer_t *b, int rc, const char *where)
+{
+    if (b) b->last_error = rc;
+    if (rc != MYERS_SUCCESS && (b == NULL || b->exit_on_error)) {
+        fprintf(stderr, "%s in %s\n", myersGetErrorString(rc), where);
+        exit(EXIT_FAILURE);
+    }
+}
+
+/* ---------------------------------------------------------------- getters */
+
+qryEntry_t *getPEQBuffer(void *myersBuffer)        { return ((myers_buffer_t *)myersBuffer)->h_peq; }
+candInfo_t *getCandidatesBuffer(void *myersBuffer) { return ((myers_buffer_t *)myersBuffer)->h_cands; }
+qryInfo_t  *getPEQInfoBuffer(void *myersBuffer)    { return ((myers_buffer_t *)myersBuffer)->h_qinfo; }
+resEntry_t *getResultsBuffer(void *myersBuffer)    { return ((myers_buffer_t *)myersBuffer)->h_results; }
+uint32_t getMaxPEQEntries(void *myersBuffer)       { return ((myers_buffer_t *)myersBuffer)->max_peq_entries; }
+uint32_t getMaxCandidates(void *myersBuffer)       { return ((myers_buffer_t *)myersBuffer)->max_candidates; }
+uint32_t getMaxQueries(void *myersBuffer)          { return ((myers_buffer_t *)myersBuffer)->max_queries; }
+uint32_t myersGetNumBuffers(void *myersBuffer)     { return ((myers_buffer_t *)myersBuffer)->num_buffers; }
+uint64_t myersGetReferenceSize(void *myersBuffer)  { return ((myers_buffer_t *)myersBuffer)->ctx->ref_size; }
+int myersGetLastError(void *myersBuffer)           { return myersBuffer ? ((myers_buffer_t *)myersBuffer)->last_error : MYERS_E_INVALID_ARGUMENT; }
+void *myersGetStream(void *myersBuffer)            { return (void *)((myers_buffer_t *)myersBuffer)->stream; }
+
+void myersDefaultConfig(myersConfig_t *cfg)
+{
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->structSize = sizeof(*cfg);
+    cfg->device = 0;
+    cfg->exitOnError = 1;
+    cfg->bufferBytes = 0;
+}
+
+/* ------------------------------------------------------------ bin table */
+
+static uint32_t variant_cost(const myers_variant_desc_t *v)
+{
+    /* VALU work per candidate column is ~10 ops per u32 word; lanes > 1 add
+     * a cross-lane carry/shift pass that costs roughly a fifth more. */
+    uint32_t base = v->total_words * 10u + 12u * v->lanes;
+    return v->lanes > 1u ? base + base / 5u : base;
+}
+
+/* Default length-bin table: for w query words pick the variant that covers
+ * them with the least estimated VALU work per column. */
+static void default_bin_table(myers_ctx_t *ctx)
+{
+    for (uint32_t w = 0; w <= MAX_QUERY_WORDS; w++) {
+        uint32_t best = 0, best_cost = 0xFFFFFFFFu;
+        for (uint32_t v = 0; v < ctx->num_variants; v++) {
+            const myers_variant_desc_t *d = &ctx->variants[v];
+            if (d->total_words < (w ? w : 1u)) continue;
+            uint32_t cost = variant_cost(d);
+            if (cost < best_cost) { best_cost = cost; best = v; }
+        }
+        ctx->bin_of_words[w] = (uint8_t)best;
+    }
+}
+
+static uint32_t max_supported_words(const myers_ctx_t *ctx)
+{
+    uint32_t m = 0;
+    for (uint32_t v = 0; v < ctx->num_variants; v++)
+        if (ctx->variants[v].total_words > m) m = ctx->variants[v].total_words;
+    return m < MAX_QUERY_WORDS ? m : MAX_QUERY_WORDS;
+}
+
+/* ------------------------------------------------------------- reference */
+
+static int read_reference_file(const char *fn, uint32_t **words, uint64_t *num_words, uint64_t *size)
+{
+    FILE *fp = fopen(fn, "rb");
+    if (fp == NULL) return MYERS_E_OPENING_FILE;
+    uint32_t hdr[2];
+    if (fread(hdr, sizeof(uint32_t), 2, fp) != 2) { fclose(fp); return MYERS_E_READING_FILE; }
+    uint32_t *w = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)hdr[0] + 1u));
+    if (w == NULL) { fclose(fp); return MYERS_E_ALLOCATE_MEM; }
+    if (fread(w, sizeof(uint32_t), hdr[0], fp) != hdr[0]) { free(w); fclose(fp); return MYERS_E_READING_FILE; }
+    fclose(fp);
+    *words = w;
+    *num_words = hdr[0];
+    *size = hdr[1];
+    return MYERS_SUCCESS;
+}
+
+static int upload_reference(myers_ctx_t *ctx, const uint32_t *words, uint64_t num_words, uint64_t size)
+{
+    if (num_words + GUARD_WORDS >= 0xFFFFFFF0ull) return MYERS_E_REFERENCE_TOO_LARGE;
+    if (size > num_words * 8u) return MYERS_E_INVALID_ARGUMENT;
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t bytes = (num_words + GUARD_WORDS) * sizeof(uint32_t);
+    if (bytes > free_b) return MYERS_E_INSUFFICIENT_MEM_GPU;
+    HIP_TRY(hipMalloc((void **)&ctx->d_ref, bytes));
+    /* Stream the words through a pinned staging buffer. */
+    void *stage = NULL;
+    size_t stage_bytes = STAGE_BYTES;
+    if (stage_bytes > num_words * sizeof(uint32_t)) stage_bytes = num_words * sizeof(uint32_t) + 4u;
+    HIP_TRY(hipHostMalloc(&stage, stage_bytes, hipHostMallocDefault));
+    uint64_t done = 0;
+    int rc = MYERS_SUCCESS;
+    while (done < num_words && rc == MYERS_SUCCESS) {
+        uint64_t n = stage_bytes / sizeof(uint32_t);
+        if (n > num_words - done) n = num_words - done;
+        memcpy(stage, words + done, n * sizeof(uint32_t));
+        rc = hip_fail(hipMemcpy(ctx->d_ref + done, stage, n * sizeof(uint32_t), hipMemcpyHostToDevice),
+                      "hipMemcpy(reference)", __FILE__, __LINE__);
+        done += n;
+    }
+    hipHostFree(stage);
+    if (rc) return rc;
+    HIP_TRY(hipMemsetD32((hipDeviceptr_t)(ctx->d_ref + num_words), (int)ALL_N_WORD, GUARD_WORDS));
+    HIP_TRY(hipDeviceSynchronize());
+    ctx->ref_words = num_words;
+    ctx->ref_words_alloc = num_words + GUARD_WORDS;
+    ctx->ref_size = size;
+    return MYERS_SUCCESS;
+}
+
+/* ---------------------------------------------------------------- buffers */
+
+static void free_buffer(myers_buffer_t *b)
+{
+    if (b == NULL) return;
+    if (b->h_peq) hipHostFree(b->h_peq);
+    if (b->h_qinfo) hipHostFree(b->h_qinfo);
+    if (b->h_cands) hipHostFree(b->h_cands);
+    if (b->h_results) hipHostFree(b->h_results);
+    if (b->h_err) hipHostFree(b->h_err);
+    if (b->d_peq) hipFree(b->d_peq);
+    if (b->d_qinfo) hipFree(b->d_qinfo);
+    if (b->d_cands) hipFree(b->d_cands);
+    if (b->d_results) hipFree(b->d_results);
+    if (b->d_order) hipFree(b->d_order);
+    if (b->d_work) hipFree(b->d_work);
+    if (b->ev_batch_start) hipEventDestroy(b->ev_batch_start);
+    if (b->ev_batch_end) hipEventDestroy(b->ev_batch_end);
+    if (b->ev_kernel_start) hipEventDestroy(b->ev_kernel_start);
+    if (b->ev_kernel_end) hipEventDestroy(b->ev_kernel_end);
+    if (b->stream) hipStreamDestroy(b->stream);
+    free(b);
+}
+
+static int init_buffer(myers_buffer_t *b)
+{
+    const size_t peq_b = (size_t)b->max_peq_entries * sizeof(qryEntry_t);
+    const size_t qi_b = (size_t)b->max_queries * sizeof(qryInfo_t);
+    const size_t ca_b = (size_t)b->max_candidates * sizeof(candInfo_t);
+    const size_t re_b = (size_t)b->max_candidates * sizeof(resEntry_t);
+    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HIP_TRY(hipHostMalloc((void **)&b->h_peq, peq_b, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&b->h_qinfo, qi_b, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&b->h_cands, ca_b, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&b->h_results, re_b, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&b->h_err, sizeof(uint32_t), hipHostMallocDefault));
+    HIP_TRY(hipMalloc((void **)&b->d_peq, peq_b));
+    HIP_TRY(hipMalloc((void **)&b->d_qinfo, qi_b));
+    HIP_TRY(hipMalloc((void **)&b->d_cands, ca_b));
+    HIP_TRY(hipMalloc((void **)&b->d_results, re_b));
+    HIP_TRY(hipMalloc((void **)&b->d_order, (size_t)b->max_candidates * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc((void **)&b->d_work, (5u * MYERS_MAX_VARIANTS + 1u) * sizeof(uint32_t)));
+    HIP_TRY(hipEventCreate(&b->ev_batch_start));
+    HIP_TRY(hipEventCreate(&b->ev_batch_end));
+    HIP_TRY(hipEventCreate(&b->ev_kernel_start));
+    HIP_TRY(hipEventCreate(&b->ev_kernel_end));
+    memset(b->h_results, 0, re_b);
+    *b->h_err = 0;
+    b->stats.structSize = sizeof(myersStats_t);
+    return MYERS_SUCCESS;
+}
+
+static int create_context(myers_ctx_t **out, int device)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MYERS_E_NO_DEVICE;
+    if (device < 0 || device >= ndev) return MYERS_E_INVALID_ARGUMENT;
+    HIP_TRY(hipSetDevice(device));
+    myers_ctx_t *ctx = (myers_ctx_t *)calloc(1, sizeof(myers_ctx_t));
+    if (ctx == NULL) return MYERS_E_ALLOCATE_MEM;
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { free(ctx); return MYERS_E_HIP; }
+    ctx->num_cus = prop.multiProcessorCount;
+    ctx->num_variants = myers_num_variants();
+    for (uint32_t v = 0; v < ctx->num_variants; v++) {
+        myers_variant_desc(v, &ctx->variants[v]);
+        int occ = 0;
+        if (myers_variant_occupancy(v, &occ) != 0 || occ <= 0) occ = 1;
+        ctx->occupancy[v] = occ;
+    }
+    default_bin_table(ctx);
+    if (hipMalloc((void **)&ctx->d_bin_of_words, sizeof(ctx->bin_of_words)) != hipSuccess) {
+        free(ctx);
+        return MYERS_E_HIP;
+    }
+    if (hipMemcpy(ctx->d_bin_of_words, ctx->bin_of_words, sizeof(ctx->bin_of_words),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(ctx->d_bin_of_words);
+        free(ctx);
+        return MYERS_E_HIP;
+    }
+    *out = ctx;
+    return MYERS_SUCCESS;
+}
+
+static void free_context(myers_ctx_t *ctx)
+{
+    if (ctx == NULL) return;
+    if (ctx->d_ref) hipFree(ctx->d_ref);
+    if (ctx->d_bin_of_words) hipFree(ctx->d_bin_of_words);
+    free(ctx);
+}
+
+/* Capacities from the per-buffer byte budget, with the reference's formulas
+ * (src-api/myersGPU.c:312-330). */
+static int size_buffers(myers_ctx_t *ctx, uint32_t num_buffers, int32_t avg_query, int32_t cands_per_query,
+                        uint64_t buffer_bytes, uint32_t *max_cands, uint32_t *max_queries, uint32_t *max_peq)
+{
+    if (avg_query <= 0) avg_query = 1024;
+    if (cands_per_query <= 0) cands_per_query = 1;
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    uint64_t budget = buffer_bytes;
+    if (budget == 0) {
+        budget = (uint64_t)((double)free_b * 0.95) / num_buffers;
+        if (budget > DEFAULT_BUFFER_CAP) budget = DEFAULT_BUFFER_CAP;
+    }
+    if (budget * num_buffers > (uint64_t)free_b) return MYERS_E_INSUFFICIENT_MEM_GPU;
+    const uint32_t entries = ((uint32_t)avg_query + 127u) / 128u;
+    const double per_cand = ((double)entries * sizeof(qryEntry_t) + sizeof(qryInfo_t)) / cands_per_query
+                            + sizeof(candInfo_t) + sizeof(resEntry_t) + sizeof(uint32_t);
+    double mc = (double)budget / per_cand;
+    if (mc > 4.0e9) mc = 4.0e9;
+    *max_cands = (uint32_t)mc;
+    *max_queries = *max_cands / (uint32_t)cands_per_query;
+    double mp = (double)(*max_queries) * entries;
+    if (mp > 4.0e9) mp = 4.0e9;
+    *max_peq = (uint32_t)mp;
+    (void)ctx;
+    if (*max_cands < 1u || *max_queries < 1u) return MYERS_E_INSUFFICIENT_MEM_GPU;
+    return MYERS_SUCCESS;
+}
+
+static int init_common(void ***out, uint32_t num_buffers, const uint32_t *words, uint64_t num_words,
+                       uint64_t size, int32_t avg_query, int32_t cands_per_query, const myersConfig_t *cfg_in)
+{
+    if (out == NULL || num_buffers == 0 || words == NULL) return MYERS_E_INVALID_ARGUMENT;
+    myersConfig_t cfg;
+    myersDefaultConfig(&cfg);
+    if (cfg_in) cfg = *cfg_in;
+    *out = NULL;
+    myers_ctx_t *ctx = NULL;
+    int rc = create_context(&ctx, cfg.device);
+    if (rc) return rc;
+    rc = upload_reference(ctx, words, num_words, size);
+    if (rc) { free_context(ctx); return rc; }
+    uint32_t mc = 0, mq = 0, mp = 0;
+    rc = size_buffers(ctx, num_buffers, avg_query, cands_per_query, cfg.bufferBytes, &mc, &mq, &mp);
+    if (rc) { free_context(ctx); return rc; }
+    myers_buffer_t **bufs = (myers_buffer_t **)calloc(num_buffers, sizeof(myers_buffer_t *));
+    if (bufs == NULL) { free_context(ctx); return MYERS_E_ALLOCATE_MEM; }
+    for (uint32_t i = 0; i < num_buffers && rc == MYERS_SUCCESS; i++) {
+        myers_buffer_t *b = (myers_buffer_t *)calloc(1, sizeof(myers_buffer_t));
+        if (b == NULL) { rc = MYERS_E_ALLOCATE_MEM; break; }
+        bufs[i] = b;
+        b->ctx = ctx;
+        b->num_buffers = num_buffers;
+        b->id_buffer = i;
+        b->max_candidates = mc;
+        b->max_queries = mq;
+        b->max_peq_entries = mp;
+        b->exit_on_error = cfg.exitOnError;
+        rc = init_buffer(b);
+    }
+    if (rc) {
+        for (uint32_t i = 0; i < num_buffers; i++) free_buffer(bufs[i]);
+        free(bufs);
+        free_context(ctx);
+        return rc;
+    }
+    *out = (void **)bufs;
+    return MYERS_SUCCESS;
+}
+
+int myersInitFromMemory(void ***myersBuffer, uint32_t numBuffers, const uint32_t *refWords,
+                        uint64_t numWords, uint64_t refSize, int32_t averageQuerySize,
+                        int32_t candidatesPerQuery, const myersConfig_t *cfg)
+{
+    return init_common(myersBuffer, numBuffers, refWords, numWords, refSize, averageQuerySize,
+                       candidatesPerQuery, cfg);
+}
+
+int myersInitEx(void ***myersBuffer, uint32_t numBuffers, const char *referenceFileName,
+                int32_t averageQuerySize, int32_t candidatesPerQuery, const myersConfig_t *cfg)
+{
+    if (referenceFileName == NULL) return MYERS_E_INVALID_ARGUMENT;
+    uint32_t *words = NULL;
+    uint64_t nw = 0, size = 0;
+    int rc = read_reference_file(referenceFileName, &words, &nw, &size);
+    if (rc) return rc;
+    rc = init_common(myersBuffer, numBuffers, words, nw, size, averageQuerySize, candidatesPerQuery, cfg);
+    free(words);
+    return rc;
+}
+
+int initMyers(void ***myersBuffer, uint32_t numBuffers, const char *referenceFileName,
+              int32_t averageQuerySize, int32_t candidatesPerQuery)
+{
+    return myersInitEx(myersBuffer, numBuffers, referenceFileName, averageQuerySize,
+                       candidatesPerQuery, NULL);
+}
+
+int myersEnd(void ***myersBuffer)
+{
+    if (myersBuffer == NULL || *myersBuffer == NULL) return MYERS_E_INVALID_ARGUMENT;
+    myers_buffer_t **bufs = (myers_buffer_t **)(*myersBuffer);
+    myers_ctx_t *ctx = bufs[0]->ctx;
+    const uint32_t n = bufs[0]->num_buffers;
+    hipSetDevice(ctx->device);
+    int rc = hip_fail(hipDeviceSynchronize(), "hipDeviceSynchronize", __FILE__, __LINE__);
+    for (uint32_t i = 0; i < n; i++) free_buffer(bufs[i]);
+    free(bufs);
+    free_context(ctx);   /* the shared reference is freed once */
+    *myersBuffer = NULL;
+    return rc;
+}
+
+void endMyers(void ***myersBuffer)
+{
+    int rc = myersEnd(myersBuffer);
+    if (rc) {
+        fprintf(stderr, "%s in endMyers\n", myersGetErrorString(rc));
+        exit(EXIT_FAILURE);
+    }
+}
+
+/* ----------------------------------------------------------------- launch */
+
+typedef struct {
+    uint64_t present;      /* bitmask of variants present */
+    uint32_t num_present;
+    uint32_t single;       /* the variant when num_present == 1 */
+    int      too_long;
+    uint64_t cells_per_cand_sum_unused;
+} bin_scan_t;
+
+static void scan_qinfo(const myers_ctx_t *ctx, const qryInfo_t *qinfo, uint32_t nq, bin_scan_t *s)
+{
+    const uint32_t maxw = max_supported_words(ctx);
+    memset(s, 0, sizeof(*s));
+    for (uint32_t q = 0; q < nq; q++) {
+        uint32_t w = (qinfo[q].size + 31u) >> 5;
+        if (w > maxw) { s->too_long = 1; continue; }
+        s->present |= 1ull << ctx->bin_of_words[w];
+    }
+    for (uint32_t v = 0; v < ctx->num_variants; v++)
+        if (s->present & (1ull << v)) { s->num_present++; s->single = v; }
+}
+
+static uint32_t grid_for(const myers_ctx_t *ctx, uint32_t v, uint32_t count, int persistent)
+{
+    const uint32_t slots = ctx->variants[v].slots;
+    uint64_t chunks = ((uint64_t)count + slots - 1u) / slots;
+    if (chunks == 0) chunks = 1;
+    if (persistent) {
+        uint64_t resident = (uint64_t)ctx->num_cus * (uint64_t)ctx->occupancy[v];
+        /* keep the grid a multiple of 8 (one share per XCD) */
+        resident = (resident + 7u) & ~7ull;
+        if (chunks > resident) chunks = resident;
+    }
+    if (chunks > 0x7FFFFFFFull) chunks = 0x7FFFFFFFull;
+    return (uint32_t)chunks;
+}
+
+/* Enqueue the verification of num_cands candidates on b->stream. */
+static int enqueue_verify(myers_buffer_t *b, const void *d_peq, uint32_t num_entries, const void *d_qinfo,
+                          const void *d_cands, uint32_t num_cands, float distance, uint32_t flags,
+                          void *d_results, const qryInfo_t *h_qinfo, uint32_t num_queries)
+{
+    myers_ctx_t *ctx = b->ctx;
+    const uint32_t mode = flags & 1u;
+    bin_scan_t scan;
+    if (h_qinfo) {
+        scan_qinfo(ctx, h_qinfo, num_queries, &scan);
+    } else {
+        memset(&scan, 0, sizeof(scan));
+        scan.present = ctx->num_variants >= 64 ? ~0ull : ((1ull << ctx->num_variants) - 1u);
+        scan.num_present = ctx->num_variants;
+        scan.too_long = 1;   /* unknown: let the device check */
+    }
+    myers_verify_args_t a;
+    memset(&a, 0, sizeof(a));
+    a.ref = ctx->d_ref;
+    a.ref_words = ctx->ref_words_alloc;
+    a.ref_size = ctx->ref_size;
+    a.peq = (const uint32_t *)d_peq;
+    a.qinfo = (const uint32_t *)d_qinfo;
+    a.num_peq_entries = num_entries;
+    a.num_queries = num_queries;
+    a.cands = d_cands;
+    a.mode = mode;
+    a.distance = distance;
+    a.results = d_results;
+
+    b->stats.launches = 0;
+    b->stats.variantsUsed = 0;
+    b->binned = 0;
+    if (num_cands == 0) {
+        HIP_TRY(hipEventRecord(b->ev_kernel_start, b->stream));
+        HIP_TRY(hipEventRecord(b->ev_kernel_end, b->stream));
+        return MYERS_SUCCESS;
+    }
+    if (scan.num_present == 1 && !scan.too_long) {
+        const uint32_t v = scan.single;
+        a.order = NULL;
+        a.bin_ranges = NULL;
+        a.start = 0;
+        a.count = num_cands;
+        HIP_TRY(hipEventRecord(b->ev_kernel_start, b->stream));
+        int rc = myers_launch_verify(v, &a, grid_for(ctx, v, num_cands, 0), b->stream);
+        if (rc) return hip_fail((hipError_t)rc, "verify launch", __FILE__, __LINE__);
+        HIP_TRY(hipEventRecord(b->ev_kernel_end, b->stream));
+        b->stats.launches = 1;
+        b->stats.variantsUsed = 1;
+        return MYERS_SUCCESS;
+    }
+    /* device-side binning */
+    uint32_t *counts = b->d_work;
+    uint32_t *cursors = counts + MYERS_MAX_VARIANTS;
+    uint32_t *ranges = cursors + MYERS_MAX_VARIANTS;
+    uint32_t *err = ranges + 2u * MYERS_MAX_VARIANTS;
+    int rc = myers_launch_binning((const uint32_t *)d_qinfo, num_queries, d_cands, num_cands, ctx->d_bin_of_words,
+                                  max_supported_words(ctx), ctx->num_variants, counts, cursors, ranges,
+                                  b->d_order, err, d_results, b->stream);
+    if (rc) return hip_fail((hipError_t)rc, "binning launch", __FILE__, __LINE__);
+    HIP_TRY(hipMemcpyAsync(b->h_err, err, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
+    b->binned = 1;
+    a.order = b->d_order;
+    a.bin_ranges = ranges;
+    HIP_TRY(hipEventRecord(b->ev_kernel_start, b->stream));
+    /* longest queries first: they have the fewest workgroups per launch */
+    for (int v = (int)ctx->num_variants - 1; v >= 0; v--) {
+        if (!(scan.present & (1ull << v))) continue;
+        a.variant = (uint32_t)v;
+        rc = myers_launch_verify((uint32_t)v, &a, grid_for(ctx, (uint32_t)v, num_cands, 1), b->stream);
+        if (rc) return hip_fail((hipError_t)rc, "verify launch", __FILE__, __LINE__);
+        b->stats.launches++;
+        b->stats.variantsUsed++;
+    }
+    HIP_TRY(hipEventRecord(b->ev_kernel_end, b->stream));
+    return MYERS_SUCCESS;
+}
+
+static uint64_t nominal_cells(const qryInfo_t *qinfo, const candInfo_t *cands, uint32_t nc, float d)
+{
+    uint64_t cells = 0;
+    for (uint32_t i = 0; i < nc; i++) {
+        uint32_t m = qinfo[cands[i].query].size;
+        volatile float f = (float)m * (1.0f + 2.0f * d);
+        cells += (uint64_t)m * (uint64_t)(uint32_t)f;
+    }
+    return cells;
+}
+
+int myersSendEx(void *myersBuffer, float distance, uint32_t numPEQEntries, uint32_t numQueries,
+                uint32_t numCandidates, uint32_t flags)
+{
+    myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
+    if (b == NULL) return MYERS_E_INVALID_ARGUMENT;
+    if (flags & ~(1u | MYERS_ENGINE_MASK)) return (b->last_error = MYERS_E_INVALID_ARGUMENT);
+    if (numPEQEntries > b->max_peq_entries || numQueries > b->max_queries || numCandidates > b->max_candidates)
+        return (b->last_error = MYERS_E_CAPACITY);
+    int rc = hip_fail(hipSetDevice(b->ctx->device), "hipSetDevice", __FILE__, __LINE__);
+    if (rc) return (b->last_error = rc);
+    /* candidates must name a query of this batch */
+    for (uint32_t i = 0; i < numCandidates; i++)
+        if (b->h_cands[i].query >= numQueries) return (b->last_error = MYERS_E_INVALID_ARGUMENT);
+    *b->h_err = 0;
+    b->num_candidates = numCandidates;
+    hipStream_t s = b->stream;
+#define SEND_TRY(call) do { rc = hip_fail((call), #call, __FILE__, __LINE__); if (rc) return (b->last_error = rc); } while (0)
+    SEND_TRY(hipEventRecord(b->ev_batch_start, s));
+    if (numPEQEntries)
+        SEND_TRY(hipMemcpyAsync(b->d_peq, b->h_peq, (size_t)numPEQEntries * sizeof(qryEntry_t), hipMemcpyHostToDevice, s));
+    if (numQueries)
+        SEND_TRY(hipMemcpyAsync(b->d_qinfo, b->h_qinfo, (size_t)numQueries * sizeof(qryInfo_t), hipMemcpyHostToDevice, s));
+    if (numCandidates)
+        SEND_TRY(hipMemcpyAsync(b->d_cands, b->h_cands, (size_t)numCandidates * sizeof(candInfo_t), hipMemcpyHostToDevice, s));
+    rc = enqueue_verify(b, b->d_peq, numPEQEntries, b->d_qinfo, b->d_cands, numCandidates, distance, flags,
+                        b->d_results, b->h_qinfo, numQueries);
+    if (rc) return (b->last_error = rc);
+    if (numCandidates)
+        SEND_TRY(hipMemcpyAsync(b->h_results, b->d_results, (size_t)numCandidates * sizeof(resEntry_t), hipMemcpyDeviceToHost, s));
+    SEND_TRY(hipEventRecord(b->ev_batch_end, s));
+#undef SEND_TRY
+    b->stats.cells = nominal_cells(b->h_qinfo, b->h_cands, numCandidates, distance);
+    b->stats.candidates = numCandidates;
+    b->pending = 1;
+    b->last_error = MYERS_SUCCESS;
+    return MYERS_SUCCESS;
+}
+
+static int finish_batch(myers_buffer_t *b, int with_batch_events)
+{
+    int rc = hip_fail(hipStreamSynchronize(b->stream), "hipStreamSynchronize", __FILE__, __LINE__);
+    if (rc) return rc;
+    if (!b->pending) return MYERS_SUCCESS;
+    b->pending = 0;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, b->ev_kernel_start, b->ev_kernel_end) == hipSuccess) {
+        b->stats.kernelMs = ms;
+        b->stats.kernelMsTotal += ms;
+    }
+    b->stats.kernelLaunchesTotal += b->stats.launches;
+    b->stats.batchesTotal++;
+    if (with_batch_events && hipEventElapsedTime(&ms, b->ev_batch_start, b->ev_batch_end) == hipSuccess)
+        b->stats.batchMs = ms;
+    b->stats.binned = (uint32_t)b->binned;
+    if (b->binned && (*b->h_err & 1u)) return MYERS_E_QUERY_TOO_LONG;
+    if (b->binned && (*b->h_err & 2u)) return MYERS_E_INVALID_ARGUMENT;
+    return MYERS_SUCCESS;
+}
+
+int myersReceiveEx(void *myersBuffer)
+{
+    myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
+    if (b == NULL) return MYERS_E_INVALID_ARGUMENT;
+    int rc = hip_fail(hipSetDevice(b->ctx->device), "hipSetDevice", __FILE__, __LINE__);
+    if (rc == MYERS_SUCCESS) rc = finish_batch(b, 1);
+    b->last_error = rc;
+    return rc;
+}
+
+void sendMyersBuffer(void *myersBuffer, float distance, uint32_t numPEQEntries, uint32_t numQueries,
+                     uint32_t numCandidates)
+{
+    int rc = myersSendEx(myersBuffer, distance, numPEQEntries, numQueries, numCandidates, 0u);
+    report((myers_buffer_t *)myersBuffer, rc, "sendMyersBuffer");
+}
+
+void receiveMyersBuffer(void *myersBuffer)
+{
+    int rc = myersReceiveEx(myersBuffer);
+    report((myers_buffer_t *)myersBuffer, rc, "receiveMyersBuffer");
+}
+
+/* ------------------------------------------------------- device batches */
+
+int myersVerifyDevice(void *myersBuffer, const void *d_peq, uint32_t numPEQEntries, const void *d_qinfo,
+                      uint32_t numQueries, const void *d_candidates, uint32_t numCandidates,
+                      float distance, uint32_t flags, void *d_results, const qryInfo_t *h_qinfo)
+{
+    myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
+    if (b == NULL) return MYERS_E_INVALID_ARGUMENT;
+    if ((flags & ~(1u | MYERS_ENGINE_MASK)) || (numCandidates && (!d_peq || !d_qinfo || !d_candidates || !d_results)))
+        return (b->last_error = MYERS_E_INVALID_ARGUMENT);
+    if (numCandidates > b->max_candidates) return (b->last_error = MYERS_E_CAPACITY);
+    int rc = hip_fail(hipSetDevice(b->ctx->device), "hipSetDevice", __FILE__, __LINE__);
+    if (rc) return (b->last_error = rc);
+    *b->h_err = 0;
+    rc = enqueue_verify(b, d_peq, numPEQEntries, d_qinfo, d_candidates, numCandidates, distance, flags,
+                        d_results, h_qinfo, numQueries);
+    b->stats.cells = 0;
+    b->stats.candidates = numCandidates;
+    b->pending = (rc == MYERS_SUCCESS);
+    b->last_error = rc;
+    return rc;
+}
+
+int myersSynchronize(void *myersBuffer)
+{
+    myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
+    if (b == NULL) return MYERS_E_INVALID_ARGUMENT;
+    int rc = hip_fail(hipSetDevice(b->ctx->device), "hipSetDevice", __FILE__, __LINE__);
+    if (rc == MYERS_SUCCESS) rc = finish_batch(b, 0);
+    b->last_error = rc;
+    return rc;
+}
+
+int myersGetStats(void *myersBuffer, myersStats_t *out)
+{
+    myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
+    if (b == NULL || out == NULL) return MYERS_E_INVALID_ARGUMENT;
+    *out = b->stats;
+    return MYERS_SUCCESS;
+}
+
+void myersResetStats(void *myersBuffer)
+{
+    myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
+    if (b == NULL) return;
+    b->stats.kernelMsTotal = 0.0;
+    b->stats.kernelLaunchesTotal = 0;
+    b->stats.batchesTotal = 0;
+}
+
+/* -------------------------------------------------------------- helpers */
+
+int myersDeviceMalloc(void **ptr, size_t bytes)
+{
+    if (ptr == NULL) return MYERS_E_INVALID_ARGUMENT;
+    return hip_fail(hipMalloc(ptr, bytes ? bytes : 1u), "hipMalloc", __FILE__, __LINE__);
+}
+
+int myersDeviceFree(void *ptr)
+{
+    return hip_fail(hipFree(ptr), "hipFree", __FILE__, __LINE__);
+}
+
+int myersMemcpyHtoD(void *dst, const void *src, size_t bytes)
+{
+    return hip_fail(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy(HtoD)", __FILE__, __LINE__);
+}
+
+int myersMemcpyDtoH(void *dst, const void *src, size_t bytes)
+{
+    return hip_fail(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy(DtoH)", __FILE__, __LINE__);
+}
+
+uint32_t myersNumVariants(void) { return myers_num_variants(); }
+
+int myersGetVariantInfo(void *myersBuffer, uint32_t variant, myersVariantInfo_t *out)
+{
+    myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
+    if (b == NULL || out == NULL || variant >= b->ctx->num_variants) return MYERS_E_INVALID_ARGUMENT;
+    const myers_variant_desc_t *d = &b->ctx->variants[variant];
+    out->lanes = d->lanes;
+    out->words = d->words;
+    out->maxQuerySize = 32u * d->total_words;
+    out->candidatesPerWave = d->slots;
+    out->ldsBytes = d->lds_bytes;
+    out->workgroupsPerCU = b->ctx->occupancy[variant];
+    return MYERS_SUCCESS;
+}
+
+int myersGetBinTable(void *myersBuffer, uint8_t *bin, uint32_t maxWords)
+{
+    myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
+    if (b == NULL || bin == NULL) return MYERS_E_INVALID_ARGUMENT;
+    for (uint32_t w = 0; w <= maxWords && w <= MAX_QUERY_WORDS; w++) bin[w] = b->ctx->bin_of_words[w];
+    return MYERS_SUCCESS;
+}
+
+int myersSetBinTable(void *myersBuffer, const uint8_t *bin, uint32_t maxWords)
+{
+    myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
+    if (b == NULL || bin == NULL || maxWords > MAX_QUERY_WORDS) return MYERS_E_INVALID_ARGUMENT;
+    myers_ctx_t *ctx = b->ctx;
+    for (uint32_t w = 0; w <= maxWords; w++) {
+        if (bin[w] >= ctx->num_variants) return MYERS_E_INVALID_ARGUMENT;
+        if (ctx->variants[bin[w]].total_words < (w ? w : 1u)) return MYERS_E_INVALID_ARGUMENT;
+    }
+    int rc = hip_fail(hipSetDevice(ctx->device), "hipSetDevice", __FILE__, __LINE__);
+    if (rc) return rc;
+    /* the table is shared by every buffer: wait for in-flight work first */
+    rc = hip_fail(hipDeviceSynchronize(), "hipDeviceSynchronize", __FILE__, __LINE__);
+    if (rc) return rc;
+    for (uint32_t w = 0; w <= maxWords; w++) ctx->bin_of_words[w] = bin[w];
+    return hip_fail(hipMemcpy(ctx->d_bin_of_words, ctx->bin_of_words, sizeof(ctx->bin_of_words),
+                              hipMemcpyHostToDevice), "hipMemcpy(bin table)", __FILE__, __LINE__);
+}

@@ -1,0 +1,73 @@
+/*
+ * C-ABI between the C host library (myers_host.c) and the gfx950 kernels
+ * (myers_kernels.hip).  Plain C types only.
+ */
+#ifndef MYERS_KERNELS_H
+#define MYERS_KERNELS_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { MYERS_MAX_VARIANTS = 64 };
+
+/* Kernel variant: one query bit-vector of L*W u32 words split across L
+ * lanes of a wave64, W words per lane. */
+typedef struct {
+    uint32_t lanes;          /* L: lanes per candidate (1..64, power of two) */
+    uint32_t words;          /* W: u32 words per lane */
+    uint32_t total_words;    /* L*W: longest query = 32*L*W bases */
+    uint32_t slots;          /* candidates per wave = 64 / L */
+    uint32_t lds_bytes;      /* LDS per workgroup (one wave) */
+} myers_variant_desc_t;
+
+typedef struct {
+    const uint32_t *ref;         /* packed reference + guard words (all-N) */
+    uint64_t        ref_words;   /* words readable at ref (incl. guard) */
+    uint64_t        ref_size;    /* reference length in bases */
+    const uint32_t *peq;         /* qryEntry_t[] as u32[20] */
+    const uint32_t *qinfo;       /* qryInfo_t[] as u32 pairs {posEntry, size} */
+    uint32_t        num_peq_entries;
+    uint32_t        num_queries;
+    const void     *cands;       /* candInfo_t[] (16 B) */
+    const uint32_t *order;       /* slot -> candidate, NULL = identity */
+    const uint32_t *bin_ranges;  /* device u32 pairs {start, count} per variant, or NULL */
+    uint32_t        start;       /* slot range used when bin_ranges == NULL */
+    uint32_t        count;
+    uint32_t        variant;     /* row of bin_ranges */
+    uint32_t        mode;        /* 0 aligned window, 1 exact-start window */
+    float           distance;
+    uint32_t        pad0;
+    void           *results;     /* resEntry_t[] in caller order */
+} myers_verify_args_t;
+
+/* Variant table compiled into the library. */
+uint32_t myers_num_variants(void);
+int      myers_variant_desc(uint32_t v, myers_variant_desc_t *out);
+/* Workgroups of variant v resident per CU (occupancy), 0 on error. */
+int      myers_variant_occupancy(uint32_t v, int *wg_per_cu);
+
+/* Verify the slots of one variant.  grid = number of one-wave workgroups;
+ * the kernel strides over chunks of `slots` candidates. */
+int myers_launch_verify(uint32_t v, const myers_verify_args_t *args, uint32_t grid,
+                        hipStream_t stream);
+
+/* Device-side length binning.  bin_of_words[w] (w = ceil(m/32), 0..max_words)
+ * gives the variant of a query of w words.  counts/offsets/cursors hold
+ * MYERS_MAX_VARIANTS u32 each; ranges gets {start, count} pairs; order gets
+ * the candidate of every slot.  A candidate whose query is longer than
+ * max_words words gets the skip sentinel in results and sets err (u32) to 1;
+ * one naming a query >= num_queries gets the sentinel and sets err to 2. */
+int myers_launch_binning(const uint32_t *qinfo, uint32_t num_queries, const void *cands, uint32_t num_cands,
+                         const uint8_t *bin_of_words, uint32_t max_words, uint32_t num_variants,
+                         uint32_t *counts, uint32_t *cursors, uint32_t *ranges, uint32_t *order,
+                         uint32_t *err, void *results, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,519 @@
+// Candidate-verification kernels for MI355X (gfx950, wave64).
+//
+// Each candidate is verified by the Myers/Hyyro bit-vector recurrence over
+// its reference window (reference kernel: src-api/coreMyersGPU-CC35.cu:110-240;
+// CPU form: src/myers-4bits.c:111-135).  Design (all integer VALU work, no
+// MFMA):
+//   * A kernel variant <L, W> gives each candidate L lanes of a wave64 and
+//     each lane W u32 words of the query bit-vector (32*L*W rows in all).
+//     L = 1 keeps the whole recurrence inside one lane (no cross-lane work);
+//     L > 1 splits long queries across lanes with a carry-lookahead over two
+//     wave ballots for the addition and a ballot shift for the Ph/Mh bits.
+//   * Query bit-vectors are top-justified: a query of m bases occupies the
+//     top m rows and the rows below it never match (their Peq bits are 0).
+//     Such rows stay at the fixed point Pv=~0, Mv=0, so they emit no
+//     horizontal delta and the query rows see the semi-global boundary
+//     D[0][j]=0 unchanged.  The score row m-1 is therefore always bit 31 of
+//     the top lane's top word, whatever m is within the variant.
+//   * Peq masks are transposed into LDS per lane as [W/4][5 bases][64 lanes]
+//     x 128-bit, so one column's Eq words are ceil(W/4) conflict-free
+//     ds_read_b128 at an address chosen by the column's base code.
+//   * The packed reference (8 bases per u32) streams through a two-word
+//     register window per lane; exact-start windows use a funnel shift.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "myers_kernels.h"
+
+namespace {
+
+struct CandRec {
+    uint32_t query;
+    uint32_t pad;
+    uint64_t position;
+};
+
+constexpr uint32_t kSkipScore = 0xFFFFFFFFu;
+
+// n = (uint32)(m * (1 + 2*d)) in float32 with round-to-nearest steps
+// (src-api/coreMyersGPU-CC35.cu:131).
+__device__ __forceinline__ uint32_t window_len(uint32_t m, float d) {
+    float f = __fmul_rn((float)m, __fadd_rn(1.0f, __fmul_rn(2.0f, d)));
+    return (uint32_t)f;
+}
+
+// Rows 5..7 of the Peq do not exist; the reference kernel's selectEq maps
+// them to row 0 (src-api/coreMyersGPU-CC35.cu:82-95).  Applied to all eight
+// nibbles of a text word at once: clear bit 3, then zero codes 5..7.
+__device__ __forceinline__ uint32_t sanitize_text(uint32_t w) {
+    uint32_t c = w & 0x77777777u;
+    uint32_t hi = (c >> 2) & 0x11111111u;
+    uint32_t lo = (c | (c >> 1)) & 0x11111111u;
+    return c & ~((hi & lo) * 7u);
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+template <int L>
+struct LaneMasks {
+    // bit i set iff lane i is the top (resp. first) lane of its group
+    static constexpr uint64_t top() {
+        uint64_t m = 0;
+        for (int i = L - 1; i < 64; i += L) m |= 1ull << i;
+        return m;
+    }
+    static constexpr uint64_t first() {
+        uint64_t m = 0;
+        for (int i = 0; i < 64; i += L) m |= 1ull << i;
+        return m;
+    }
+};
+
+template <int L, int W>
+struct Shape {
+    static constexpr int kGroups = (W + 3) / 4;        // 128-bit LDS groups per lane
+    static constexpr int kTotal = L * W;                // words per candidate
+    static constexpr int kSlots = 64 / L;               // candidates per wave
+    static constexpr int kLdsU4 = kGroups * 5 * 64;
+};
+
+// Stage the Peq words of this lane into LDS, top-justified.  Lane word k
+// holds virtual rows [32g, 32g+32) with g = (lane % L)*W + k; virtual row r
+// is query row r - pad, pad = 32*L*W - m, so lane word k is the funnel of
+// query words lo+k and lo+k+1 (lo = floor((32*g0 - pad)/32)) shifted right
+// by sh = (-pad) mod 32.  Query word j (rows 32j..32j+31) of base c is
+// bitmap[c][j%4] of entry posEntry + j/4; words outside the query read 0.
+template <int L, int W>
+__device__ __forceinline__ void stage_peq(uint4 *lds, const uint32_t *__restrict__ peq,
+                                          uint32_t pos_entry, uint32_t m, uint32_t lane) {
+    using S = Shape<L, W>;
+    uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
+    const int32_t pad = 32 * S::kTotal - (int32_t)m;
+    const int32_t nq = (int32_t)((m + 31u) >> 5);
+    const int32_t b0 = 32 * (int32_t)((lane % L) * W) - pad;
+    const int32_t lo = b0 >> 5;                        // floor division
+    const uint32_t sh = (uint32_t)b0 & 31u;
+    const uint32_t *base = peq + (size_t)pos_entry * 20u;
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+        uint32_t prev = 0;
+        {
+            const int32_t j = lo;
+            if (j >= 0 && j < nq) prev = base[(uint32_t)(j >> 2) * 20u + (uint32_t)(j & 3) + 4u * c];
+        }
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            const int32_t j = lo + k + 1;
+            uint32_t next = 0;
+            if (j >= 0 && j < nq) next = base[(uint32_t)(j >> 2) * 20u + (uint32_t)(j & 3) + 4u * c];
+            const uint32_t v = (uint32_t)((((uint64_t)next << 32) | prev) >> sh);
+            l32[(((k >> 2) * 5 + c) * 64 + lane) * 4 + (k & 3)] = v;
+            prev = next;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+template <int L, int W>
+__device__ __forceinline__ void load_eq(const uint4 *lds_lane, uint32_t row_off, uint32_t (&eq)[W]) {
+    using S = Shape<L, W>;
+    const uint4 *p = lds_lane + row_off;
+#pragma unroll
+    for (int g = 0; g < S::kGroups; g++) {
+        uint4 v = p[g * 5 * 64];
+        eq[4 * g] = v.x;
+        if (4 * g + 1 < W) eq[4 * g + 1] = v.y;
+        if (4 * g + 2 < W) eq[4 * g + 2] = v.z;
+        if (4 * g + 3 < W) eq[4 * g + 3] = v.w;
+    }
+}
+
+// Truth tables of v_bitop3_b32(a, b, c): bit (4a+2b+c) of the immediate is
+// f(a, b, c); with a = 0xF0, b = 0xCC, c = 0xAA the table is f applied
+// bitwise to those constants.
+constexpr uint32_t kXorOr = (0xF0u ^ 0xCCu) | 0xAAu;          // (a ^ b) | c
+constexpr uint32_t kOrNor = (0xF0u | ~(0xCCu | 0xAAu)) & 0xFFu;  // a | ~(b | c)
+
+// One column of the recurrence for a lane-local bit-vector (L = 1):
+//   Xv = Eq|Mv;  Xh = (((Eq&Pv)+Pv)^Pv)|Eq;  Ph = Mv|~(Xh|Pv);  Mh = Pv&Xh
+//   Ph <<= 1; Mh <<= 1 (row 0 gets 0: D[0][j] = 0);  Pv = Mh|~(Xv|Ph);  Mv = Ph&Xv
+// The carry of the addition and the bits shifted out of each word run
+// through the whole 32*W-bit vector: 10 VALU ops per word (and, add-carry,
+// 3 bitop3, 2 funnel shifts, or, bitop3, and).  Returns the top word's
+// Ph/Mh (before the shift) for the score row.
+template <int W>
+__device__ __forceinline__ void column_local(uint32_t (&Pv)[W], uint32_t (&Mv)[W],
+                                             const uint32_t (&Eq)[W], uint32_t &ph_top,
+                                             uint32_t &mh_top) {
+    uint32_t carry = 0, ph_prev = 0, mh_prev = 0;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const uint32_t pv = Pv[k], mv = Mv[k], eq = Eq[k];
+        const uint32_t xv = eq | mv;
+        uint32_t c_out;
+        const uint32_t sum = __builtin_addc(eq & pv, pv, carry, &c_out);
+        carry = c_out;
+        const uint32_t xh = __builtin_amdgcn_bitop3_b32(sum, pv, eq, kXorOr);
+        const uint32_t ph = __builtin_amdgcn_bitop3_b32(mv, xh, pv, kOrNor);
+        const uint32_t mh = pv & xh;
+        if (k == W - 1) {
+            ph_top = ph;
+            mh_top = mh;
+        }
+        const uint32_t php = k == 0 ? (ph << 1) : __builtin_amdgcn_alignbit(ph, ph_prev, 31u);
+        const uint32_t mhp = k == 0 ? (mh << 1) : __builtin_amdgcn_alignbit(mh, mh_prev, 31u);
+        ph_prev = ph;
+        mh_prev = mh;
+        Pv[k] = __builtin_amdgcn_bitop3_b32(mhp, xv, php, kOrNor);
+        Mv[k] = php & xv;
+    }
+}
+
+// One column for a bit-vector split over L lanes (W words each).  Each lane
+// adds its words with carry-in 0; the carry into every lane then follows
+// from G (lane carry-out) and P (lane sum all ones) by one 64-bit add:
+// carries = (G + (G|P)) ^ G ^ (G|P), with G and P cleared at group tops so
+// no carry crosses into the next candidate.  The Ph/Mh bit shifted out of a
+// lane's top word enters bit 0 of the next lane via a ballot shifted by one.
+template <int L, int W>
+__device__ __forceinline__ void column_group(uint32_t (&Pv)[W], uint32_t (&Mv)[W],
+                                             const uint32_t (&Eq)[W], uint32_t &ph_top,
+                                             uint32_t &mh_top) {
+    constexpr uint64_t kTop = LaneMasks<L>::top();
+    constexpr uint64_t kFirst = LaneMasks<L>::first();
+    uint32_t sum[W];
+    uint32_t carry = 0, all_ones = ~0u;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        uint32_t c_out;
+        sum[k] = __builtin_addc(Eq[k] & Pv[k], Pv[k], carry, &c_out);
+        carry = c_out;
+        all_ones &= sum[k];
+    }
+    const uint64_t G = __ballot(carry != 0u) & ~kTop;
+    const uint64_t P = __ballot(all_ones == ~0u) & ~kTop;
+    const uint64_t X = G | P;
+    const uint64_t cin = (G + X) ^ G ^ X;
+    uint32_t c = __builtin_amdgcn_inverse_ballot_w64(cin) ? 1u : 0u;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        uint32_t c_out;
+        sum[k] = __builtin_addc(sum[k], 0u, c, &c_out);
+        c = c_out;
+    }
+    uint32_t ph[W], mh[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const uint32_t xh = __builtin_amdgcn_bitop3_b32(sum[k], Pv[k], Eq[k], kXorOr);
+        ph[k] = __builtin_amdgcn_bitop3_b32(Mv[k], xh, Pv[k], kOrNor);
+        mh[k] = Pv[k] & xh;
+    }
+    ph_top = ph[W - 1];
+    mh_top = mh[W - 1];
+    const uint64_t bp = (__ballot((ph[W - 1] >> 31) != 0u) << 1) & ~kFirst;
+    const uint64_t bm = (__ballot((mh[W - 1] >> 31) != 0u) << 1) & ~kFirst;
+    const uint32_t ph_in = __builtin_amdgcn_inverse_ballot_w64(bp) ? 1u : 0u;
+    const uint32_t mh_in = __builtin_amdgcn_inverse_ballot_w64(bm) ? 1u : 0u;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const uint32_t xv = Eq[k] | Mv[k];
+        const uint32_t php = k == 0 ? ((ph[0] << 1) | ph_in) : __builtin_amdgcn_alignbit(ph[k], ph[k - 1], 31u);
+        const uint32_t mhp = k == 0 ? ((mh[0] << 1) | mh_in) : __builtin_amdgcn_alignbit(mh[k], mh[k - 1], 31u);
+        Pv[k] = __builtin_amdgcn_bitop3_b32(mhp, xv, php, kOrNor);
+        Mv[k] = php & xv;
+    }
+}
+
+template <int L, int W>
+__device__ __forceinline__ void run_column(const uint32_t (&eq)[W], uint32_t col, uint32_t ncols,
+                                           uint32_t (&Pv)[W], uint32_t (&Mv)[W], int32_t &score,
+                                           int32_t &min_score, uint32_t &min_col) {
+    uint32_t ph_top, mh_top;
+    if constexpr (L == 1) {
+        column_local<W>(Pv, Mv, eq, ph_top, mh_top);
+    } else {
+        column_group<L, W>(Pv, Mv, eq, ph_top, mh_top);
+    }
+    score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
+    const bool better = (score < min_score) && (col < ncols);
+    min_col = better ? col : min_col;
+    min_score = better ? score : min_score;
+}
+
+template <int L, int W>
+__global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
+    using S = Shape<L, W>;
+    __shared__ uint4 lds[S::kLdsU4];
+
+    const uint32_t lane = threadIdx.x;
+    uint32_t start = a.start, count = a.count;
+    if (a.bin_ranges != nullptr) {
+        start = a.bin_ranges[2 * a.variant];
+        count = a.bin_ranges[2 * a.variant + 1];
+    }
+    const uint32_t nchunks = (count + S::kSlots - 1) / S::kSlots;
+    const uint32_t *__restrict__ ref = a.ref;
+    const uint32_t last_word = (uint32_t)(a.ref_words - 1);
+    const CandRec *__restrict__ cands = static_cast<const CandRec *>(a.cands);
+    const uint2 *__restrict__ qinfo = reinterpret_cast<const uint2 *>(a.qinfo);
+    uint2 *__restrict__ results = static_cast<uint2 *>(a.results);
+    const uint4 *lds_lane = lds + lane;
+
+    // Workgroups are dispatched round-robin over the 8 XCDs; renumber them so
+    // that each XCD (own L2) takes a contiguous run of chunks, which keeps the
+    // candidates of one query -- and their Peq -- in one L2.
+    uint32_t first = blockIdx.x;
+    if ((gridDim.x & 7u) == 0u) first = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+
+    for (uint32_t chunk = first; chunk < nchunks; chunk += gridDim.x) {
+        const uint32_t slot = chunk * S::kSlots + lane / L;
+        const bool valid = slot < count;
+        uint32_t cand = 0, m = 0, pos_entry = 0;
+        uint64_t pos = 0;
+        bool sane = false;
+        if (valid) {
+            const uint32_t s = start + slot;
+            cand = a.order ? a.order[s] : s;
+            const CandRec cr = cands[cand];
+            pos = cr.position;
+            if (cr.query < a.num_queries) {
+                const uint2 qi = qinfo[cr.query];
+                pos_entry = qi.x;
+                m = qi.y;
+                // the query must fit this variant and its Peq entries the batch
+                sane = m <= 32u * S::kTotal &&
+                       (uint64_t)pos_entry + ((m + 127u) >> 7) <= (uint64_t)a.num_peq_entries;
+            }
+        }
+        if (!sane) m = 0;
+        const uint32_t n = window_len(m, a.distance);
+        // skip rule: src-api/coreMyersGPU-CC35.cu:142
+        const bool live = valid && sane && pos < a.ref_size && (a.ref_size - pos) > (uint64_t)n;
+        uint64_t nib0 = 0;
+        uint32_t ncols = 0;
+        if (live) {
+            if (a.mode == 0) {
+                // aligned window: src-api/coreMyersGPU-CC35.cu:132,144
+                nib0 = pos & ~7ull;
+                ncols = 8u * ((n >> 3) + ((n & 7u) ? 2u : 1u));
+            } else {
+                nib0 = pos;
+                ncols = n;
+            }
+        }
+        stage_peq<L, W>(lds, a.peq, pos_entry, live ? m : 0u, lane);
+
+        const uint32_t max_cols = wave_max(ncols);
+
+        uint32_t Pv[W], Mv[W];
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            Pv[k] = ~0u;
+            Mv[k] = 0u;
+        }
+        int32_t score = (int32_t)m, min_score = (int32_t)m;
+        uint32_t min_col = 0;
+
+        // Text words stream through registers two blocks (16 columns) ahead
+        // of their use; the Eq words of column j+1 are read from LDS while
+        // column j computes.  Reads past the stored reference are clamped
+        // to the last (all-N guard) word; they only feed columns that no
+        // lane of this wave keeps (col >= ncols).
+        const uint32_t w0 = (uint32_t)(nib0 >> 3);
+        const uint32_t shift = (uint32_t)(nib0 & 7u) * 4u;
+        const uint32_t tw0 = ref[min(w0, last_word)];
+        const uint32_t tw1 = ref[min(w0 + 1u, last_word)];
+        uint32_t tw2 = ref[min(w0 + 2u, last_word)];
+        uint32_t tw3 = ref[min(w0 + 3u, last_word)];
+        uint32_t text = sanitize_text(__builtin_amdgcn_alignbit(tw1, tw0, shift));
+        uint32_t text_next = sanitize_text(__builtin_amdgcn_alignbit(tw2, tw1, shift));
+        uint32_t eq[W];
+        load_eq<L, W>(lds_lane, (text & 7u) * 64u, eq);
+        const uint32_t nblocks = (max_cols + 7u) >> 3;
+        for (uint32_t b = 0; b < nblocks; b++) {
+            const uint32_t tw4 = ref[min(w0 + b + 4u, last_word)];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const uint32_t cn = (i < 7) ? ((text >> (4 * (i + 1))) & 7u) : (text_next & 7u);
+                uint32_t eqn[W];
+                load_eq<L, W>(lds_lane, cn * 64u, eqn);
+                run_column<L, W>(eq, 8u * b + (uint32_t)i, ncols, Pv, Mv, score, min_score, min_col);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int k = 0; k < W; k++) eq[k] = eqn[k];
+            }
+            text = text_next;
+            text_next = sanitize_text(__builtin_amdgcn_alignbit(tw3, tw2, shift));
+            tw2 = tw3;
+            tw3 = tw4;
+        }
+
+        if (valid && (lane % L) == (uint32_t)(L - 1)) {
+            results[cand] = live ? make_uint2(min_col, (uint32_t)min_score) : make_uint2(0u, kSkipScore);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- binning
+
+__global__ __launch_bounds__(64) void bin_reset_kernel(uint32_t *counts, uint32_t *err, uint32_t nv) {
+    for (uint32_t i = threadIdx.x; i < nv; i += blockDim.x) counts[i] = 0;
+    if (threadIdx.x == 0) *err = 0;
+}
+
+constexpr uint32_t kTooLong = 0xFFFFFFFFu;
+constexpr uint32_t kBadQuery = 0xFFFFFFFEu;
+
+__device__ __forceinline__ uint32_t variant_of(const uint2 *qinfo, uint32_t num_queries, const CandRec *cands,
+                                               uint32_t i, const uint8_t *bin_of_words, uint32_t max_words) {
+    const uint32_t q = cands[i].query;
+    if (q >= num_queries) return kBadQuery;
+    const uint32_t w = (qinfo[q].y + 31u) >> 5;
+    return w <= max_words ? (uint32_t)bin_of_words[w] : kTooLong;
+}
+
+__global__ __launch_bounds__(256) void bin_count_kernel(const uint2 *qinfo, uint32_t num_queries,
+                                                        const CandRec *cands,
+                                                        uint32_t n, const uint8_t *bin_of_words,
+                                                        uint32_t max_words, uint32_t nv,
+                                                        uint32_t *counts, uint32_t *err) {
+    __shared__ uint32_t hist[MYERS_MAX_VARIANTS];
+    for (uint32_t i = threadIdx.x; i < nv; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t v = variant_of(qinfo, num_queries, cands, i, bin_of_words, max_words);
+        if (v < nv) {
+            atomicAdd(&hist[v], 1u);
+        } else {
+            atomicOr(err, v == kBadQuery ? 2u : 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nv; i += blockDim.x)
+        if (hist[i]) atomicAdd(&counts[i], hist[i]);
+}
+
+__global__ __launch_bounds__(64) void bin_scan_kernel(const uint32_t *counts, uint32_t nv,
+                                                      uint32_t *ranges, uint32_t *cursors) {
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t v = 0; v < nv; v++) {
+            ranges[2 * v] = acc;
+            ranges[2 * v + 1] = counts[v];
+            acc += counts[v];
+            cursors[v] = 0;
+        }
+    }
+}
+
+// Scatter candidate ids into their variant's slots.  Lanes of one wave that
+// share a variant take consecutive slots in lane order (one atomic per
+// variant per wave), so the candidates of one query stay adjacent.
+__global__ __launch_bounds__(256) void bin_scatter_kernel(const uint2 *qinfo, uint32_t num_queries,
+                                                          const CandRec *cands,
+                                                          uint32_t n, const uint8_t *bin_of_words,
+                                                          uint32_t max_words, uint32_t nv,
+                                                          const uint32_t *ranges, uint32_t *cursors,
+                                                          uint32_t *order, uint2 *results) {
+    const uint32_t lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t rounds = (n + stride - 1) / stride;
+    for (uint32_t r = 0; r < rounds; r++) {
+        const uint32_t i = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        const uint32_t v = i < n ? variant_of(qinfo, num_queries, cands, i, bin_of_words, max_words) : kTooLong;
+        if (i < n && v >= nv) results[i] = make_uint2(0u, kSkipScore);
+        uint64_t pending = __ballot(v < nv);
+        while (pending) {
+            const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
+            const uint32_t vv = (uint32_t)__shfl((int)v, (int)leader);
+            const uint64_t mine = __ballot(v == vv);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&cursors[vv], (uint32_t)__builtin_popcountll(mine));
+            base = (uint32_t)__shfl((int)base, (int)leader);
+            if (v == vv) order[ranges[2 * vv] + base + (uint32_t)__builtin_popcountll(mine & below)] = i;
+            pending &= ~mine;
+        }
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ variant table
+
+// (L, W) pairs compiled into the library.  L = 1 covers queries up to
+// 32*32 = 1024 bases in one lane; L > 1 covers long reads (up to 64*12*32 =
+// 24576 bases).
+#define MYERS_VARIANT_LIST(X)                                                                    \
+    X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 5) X(1, 6) X(1, 7) X(1, 8) X(1, 10) X(1, 12) X(1, 14)   \
+    X(1, 16) X(1, 20) X(1, 24) X(1, 28) X(1, 32)                                                 \
+    X(2, 8) X(2, 12) X(2, 16) X(4, 8) X(4, 12) X(4, 16) X(8, 8) X(8, 12) X(8, 16)                \
+    X(16, 8) X(16, 12) X(16, 16) X(32, 8) X(32, 12) X(32, 16) X(64, 8) X(64, 10) X(64, 12)
+
+namespace {
+struct VariantEntry {
+    uint32_t lanes, words;
+    const void *fn;
+};
+#define MYERS_VARIANT_ENTRY(L, W) {L, W, reinterpret_cast<const void *>(&verify_kernel<L, W>)},
+const VariantEntry kVariants[] = {MYERS_VARIANT_LIST(MYERS_VARIANT_ENTRY)};
+constexpr uint32_t kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+static_assert(kNumVariants <= MYERS_MAX_VARIANTS, "variant table too large");
+}  // namespace
+
+extern "C" uint32_t myers_num_variants(void) { return kNumVariants; }
+
+extern "C" int myers_variant_desc(uint32_t v, myers_variant_desc_t *out) {
+    if (v >= kNumVariants || out == nullptr) return -1;
+    const uint32_t L = kVariants[v].lanes, W = kVariants[v].words;
+    out->lanes = L;
+    out->words = W;
+    out->total_words = L * W;
+    out->slots = 64u / L;
+    out->lds_bytes = ((W + 3u) / 4u) * 5u * 64u * 16u;
+    return 0;
+}
+
+extern "C" int myers_variant_occupancy(uint32_t v, int *wg_per_cu) {
+    if (v >= kNumVariants || wg_per_cu == nullptr) return -1;
+    int blocks = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kVariants[v].fn, 64, 0);
+    if (e != hipSuccess) return (int)e;
+    *wg_per_cu = blocks;
+    return 0;
+}
+
+extern "C" int myers_launch_verify(uint32_t v, const myers_verify_args_t *args, uint32_t grid,
+                                   hipStream_t stream) {
+    if (v >= kNumVariants || args == nullptr || grid == 0) return -1;
+    void *kargs[] = {const_cast<myers_verify_args_t *>(args)};
+    hipError_t e = hipLaunchKernel(kVariants[v].fn, dim3(grid), dim3(64), kargs, 0, stream);
+    return (int)e;
+}
+
+extern "C" int myers_launch_binning(const uint32_t *qinfo, uint32_t num_queries, const void *cands,
+                                    uint32_t num_cands,
+                                    const uint8_t *bin_of_words, uint32_t max_words,
+                                    uint32_t num_variants, uint32_t *counts, uint32_t *cursors,
+                                    uint32_t *ranges, uint32_t *order, uint32_t *err,
+                                    void *results, hipStream_t stream) {
+    if (num_variants == 0 || num_variants > MYERS_MAX_VARIANTS) return -1;
+    const uint2 *qi = reinterpret_cast<const uint2 *>(qinfo);
+    const CandRec *cr = static_cast<const CandRec *>(cands);
+    uint32_t grid = (num_cands + 255u) / 256u;
+    if (grid > 4096u) grid = 4096u;
+    if (grid == 0u) grid = 1u;
+    hipLaunchKernelGGL(bin_reset_kernel, dim3(1), dim3(64), 0, stream, counts, err, num_variants);
+    hipLaunchKernelGGL(bin_count_kernel, dim3(grid), dim3(256), 0, stream, qi, num_queries, cr, num_cands,
+                       bin_of_words, max_words, num_variants, counts, err);
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(64), 0, stream, counts, num_variants, ranges,
+                       cursors);
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid), dim3(256), 0, stream, qi, num_queries, cr, num_cands,
+                       bin_of_words, max_words, num_variants, ranges, cursors, order,
+                       static_cast<uint2 *>(results));
+    return (int)hipGetLastError();
+}

@@ -1,0 +1,221 @@
+"""Bit-exact parity of the HIP path (through the C-ABI library) with the oracle.
+
+Every comparison is exact: (column, score) per candidate.  Inputs are the
+golden fixtures written by the reference's own programs, seeded synthetic
+batches, and hand-made edge cases.
+"""
+import glob
+import os
+import threading
+
+import numpy as np
+import pytest
+
+import myers_amd as M
+import oracle_lib as O
+from test_oracle import golden_batch
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+def announce(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def synth_case(seed, ref_size, nq, **kw):
+    ref = M.synth_reference(seed, ref_size)
+    b = M.synth_batch(M.make_spec(seed + 1, nq, **kw), ref, ref_size)
+    return ref, b
+
+
+def oracle_for(ref, ref_size, b, distance, mode, max_query_len=0):
+    return O.myers_batch(ref, ref_size, b["peq"], b["qinfo"], b["cands"], distance, mode, max_query_len)
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
+def test_golden_fixtures(path):
+    announce(f"golden {os.path.basename(path)}")
+    z, peq, qinfo, cands = golden_batch(path)
+    mode = int(z["mode"])
+    with M.Myers(z["ref_words"], int(z["ref_size"]), avg_query=1024, cands_per_query=8,
+                 buffer_bytes=64 << 20) as m:
+        got = m.verify(peq, qinfo, cands, float(z["distance"]), flags=mode)
+    keep = ~z["reads_past_end"]
+    if int(z["max_query_len"]):
+        keep &= qinfo[z["cand_query"], 1] <= int(z["max_query_len"])
+    np.testing.assert_array_equal(got[keep], z["results"][keep])
+    # everything (incl. > 4096 and the past-the-end windows, which read the
+    # all-N guard) against the oracle, which has the same guard rule
+    ref_o = O.myers_batch(z["ref_words"], z["ref_size"], peq, qinfo, cands, float(z["distance"]), mode, 0)
+    np.testing.assert_array_equal(got, ref_o)
+
+
+@pytest.mark.parametrize("mode", [M.WINDOW_ALIGNED, M.WINDOW_EXACT])
+def test_config1_full(mode):
+    """4 Mbase seed-42 reference, 10K x 100 bp, 11 candidates each (110K)."""
+    announce(f"config1 mode={mode}")
+    ref, b = synth_case(42, 4_000_000, 10_000, length=100, sub=0.02, ins=0.0025, dele=0.0025,
+                        candidates=11, distance=0.1)
+    with M.Myers(ref, 4_000_000, avg_query=100, cands_per_query=11, buffer_bytes=256 << 20) as m:
+        got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=mode)
+        st = m.stats()
+    assert st.launches == 1 and st.binned == 0
+    np.testing.assert_array_equal(got, oracle_for(ref, 4_000_000, b, 0.1, mode))
+
+
+def test_mixed_lengths_binned():
+    """Config-3 style mix through device-side binning (several variants in one batch)."""
+    announce("mixed lengths")
+    ref, b = synth_case(5, 2_000_000, 600, choices=[100, 200, 400, 600, 800, 1000], err_range=(0.01, 0.05),
+                        sub_frac=0.8, poisson=11, cap=64, distance=0.1)
+    with M.Myers(ref, 2_000_000, avg_query=550, cands_per_query=11, buffer_bytes=256 << 20) as m:
+        got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1)
+        st = m.stats()
+    assert st.binned == 1 and st.launches >= 4
+    np.testing.assert_array_equal(got, oracle_for(ref, 2_000_000, b, 0.1, O.MODE_ALIGNED))
+
+
+@pytest.mark.parametrize("mode", [M.WINDOW_ALIGNED, M.WINDOW_EXACT])
+def test_long_reads(mode):
+    """Config-4 style: 2-20 kb reads, 12% error, over the multi-lane variants."""
+    announce(f"long reads mode={mode}")
+    ref, b = synth_case(11, 3_000_000, 40, log_uniform=(2000, 20000), err_range=(0.12, 0.12), sub_frac=0.5,
+                        candidates=4, distance=0.1)
+    with M.Myers(ref, 3_000_000, avg_query=8000, cands_per_query=4, buffer_bytes=256 << 20) as m:
+        got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=mode)
+    np.testing.assert_array_equal(got, oracle_for(ref, 3_000_000, b, 0.1, mode))
+
+
+def test_every_variant():
+    """Force each compiled (lanes, words) variant over queries up to its capacity."""
+    ref_size = 1_000_000
+    ref = M.synth_reference(21, ref_size)
+    with M.Myers(ref, ref_size, avg_query=2000, cands_per_query=6, buffer_bytes=256 << 20) as m:
+        variants = m.variants()
+        default = m.bin_table()
+        for v, info in enumerate(variants):
+            cap = info["maxQuerySize"]
+            announce(f"variant {v} lanes={info['lanes']} words={info['words']}")
+            lengths = sorted({max(1, cap - 32 * info["words"] * info["lanes"] + 1), cap // 2 + 1, cap - 17, cap})
+            spec = M.make_spec(100 + v, 8, choices=[x for x in lengths if x >= 1], sub=0.04, ins=0.01, dele=0.01,
+                               candidates=6, distance=0.1)
+            b = M.synth_batch(spec, ref, ref_size)
+            table = default.copy()
+            table[:] = v
+            table[(np.arange(len(table)) * 32) > cap] = default[(np.arange(len(table)) * 32) > cap]
+            m.set_bin_table(table)
+            got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1)
+            np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b, 0.1, O.MODE_ALIGNED),
+                                          err_msg=f"variant {info}")
+        m.set_bin_table(default)
+
+
+def test_edge_cases():
+    """Skip rule boundaries, m = 1, N-only text, text nibbles 5..15, windows at the reference end."""
+    announce("edge cases")
+    rng = np.random.default_rng(99)
+    size = 50_003
+    words = rng.integers(0, 2**32, size=(size + 7) // 8, dtype=np.uint64).astype(np.uint32)   # any nibble 0..15
+    words[100:140] = 0x44444444                                                                # an N run
+    qlens = [1, 2, 31, 32, 33, 64, 100, 128, 129, 500, 1024, 1025, 4096, 4097]
+    peqs, qinfo, qc, off = [], [], [], 0
+    for m_ in qlens:
+        q = rng.integers(0, 5, size=m_).astype(np.uint8)
+        p = O.build_peq(q)
+        peqs.append(p)
+        qinfo.append([off, m_])
+        off += len(p)
+    peq = np.concatenate(peqs)
+    qinfo = np.array(qinfo, dtype=np.uint32)
+    cq, cp = [], []
+    for i, m_ in enumerate(qlens):
+        n = O.window_len(m_, 0.2)
+        for p in [0, 1, 7, 8, 803, 805, size - n - 1, size - n, size - n - 9, size - 1, size, 2**40,
+                  int(rng.integers(0, size - n - 1))]:
+            cq.append(i)
+            cp.append(p)
+    cands = O.make_cands(cq, cp)
+    with M.Myers(words, size, avg_query=512, cands_per_query=13, buffer_bytes=64 << 20) as m:
+        for mode in (M.WINDOW_ALIGNED, M.WINDOW_EXACT):
+            got = m.verify(peq, qinfo, cands, 0.2, flags=mode)
+            np.testing.assert_array_equal(got, O.myers_batch(words, size, peq, qinfo, cands, 0.2, mode))
+
+
+def test_query_too_long_gets_sentinel():
+    announce("too long")
+    ref_size = 200_000
+    ref = M.synth_reference(3, ref_size)
+    rng = np.random.default_rng(1)
+    lens = [100, 24577, 300]
+    peqs, qinfo, off = [], [], 0
+    for m_ in lens:
+        p = O.build_peq(rng.integers(0, 4, size=m_).astype(np.uint8))
+        peqs.append(p)
+        qinfo.append([off, m_])
+        off += len(p)
+    cands = O.make_cands([0, 1, 2, 1], [10, 20, 30, 40])
+    with M.Myers(ref, ref_size, avg_query=8192, cands_per_query=1, buffer_bytes=64 << 20) as m:
+        peq, qinfo = np.concatenate(peqs), np.array(qinfo, dtype=np.uint32)
+        m.send(peq, qinfo, cands, 0.1)
+        with pytest.raises(M.MyersError) as e:
+            m.receive()
+        assert e.value.code == 19
+        got = m.views()["results"][:4]
+        assert tuple(got[1]) == (0, 0xFFFFFFFF) and tuple(got[3]) == (0, 0xFFFFFFFF)
+        exp = O.myers_batch(ref, ref_size, peq, qinfo, cands, 0.1, 0)
+        np.testing.assert_array_equal(got[[0, 2]], exp[[0, 2]])
+
+
+def test_device_resident_equals_buffer_api():
+    announce("device resident")
+    ref, b = synth_case(8, 1_000_000, 500, length=250, sub=0.03, ins=0.005, dele=0.005, candidates=7,
+                        distance=0.1)
+    with M.Myers(ref, 1_000_000, avg_query=250, cands_per_query=7, buffer_bytes=64 << 20) as m:
+        host = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1)
+        dp, dq, dc = M.DeviceArray(b["peq"]), M.DeviceArray(b["qinfo"]), M.DeviceArray(b["cands"])
+        dr = M.DeviceArray.empty((len(b["cands"]), 2), np.uint32)
+        for hint in (b["qinfo"], None):
+            M.verify_device(m, dp, len(b["peq"]), dq, len(b["qinfo"]), dc, len(b["cands"]), 0.1, dr, h_qinfo=hint)
+            M.synchronize(m)
+            np.testing.assert_array_equal(dr.get(), host)
+        for a in (dp, dq, dc, dr):
+            a.free()
+    np.testing.assert_array_equal(host, oracle_for(ref, 1_000_000, b, 0.1, 0))
+
+
+def test_buffers_from_threads():
+    """Several buffers driven concurrently from host threads (src-api/test.c:184-193 pattern)."""
+    announce("threads")
+    ref_size = 2_000_000
+    ref = M.synth_reference(77, ref_size)
+    batches = [M.synth_batch(M.make_spec(200 + i, 300, length=150 + 50 * i, sub=0.03, ins=0.005, dele=0.005,
+                                         candidates=11, distance=0.1), ref, ref_size) for i in range(4)]
+    out = [None] * 4
+    with M.Myers(ref, ref_size, num_buffers=4, avg_query=300, cands_per_query=11, buffer_bytes=64 << 20) as m:
+        def work(i):
+            b = batches[i]
+            for _ in range(3):
+                out[i] = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, i=i)
+        th = [threading.Thread(target=work, args=(i,)) for i in range(4)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+    for i, b in enumerate(batches):
+        np.testing.assert_array_equal(out[i], oracle_for(ref, ref_size, b, 0.1, 0))
+
+
+def test_repeat_sends_are_independent():
+    """Reusing a buffer: results depend only on the batch (the reference's numWarps grew per send)."""
+    announce("repeat sends")
+    ref, b = synth_case(9, 1_000_000, 400, choices=[100, 1000], sub=0.03, ins=0.005, dele=0.005, candidates=5,
+                        distance=0.1)
+    with M.Myers(ref, 1_000_000, avg_query=550, cands_per_query=5, buffer_bytes=64 << 20) as m:
+        first = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1)
+        half = len(b["cands"]) // 2
+        m.verify(b["peq"], b["qinfo"], b["cands"][:half], 0.1)
+        again = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1)
+    np.testing.assert_array_equal(first, again)

@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""Throughput of the candidate-verification path on one MI355X.
+
+Default workload (the headline configuration): seed-7 synthetic reference of
+3.1 Gbases (1.55 GB packed, resident in HBM), 100K reads of 1000 bp (3%
+substitutions, 0.5% insertions, 0.5% deletions, indel length geometric
+p=0.7), 11 candidates per read (1 true locus + 10 uniform), window 1200
+(distance 0.1), aligned windows (the API kernel's semantics), plain
+one-length-per-launch kernel.
+
+A step is one pass of the verification kernel over the whole batch, with
+the inputs already in HBM.  The script prints ONE JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--config 1|2|3|4] [--engine plain|persistent]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "myers-gpu_amd", "python"))
+
+import numpy as np  # noqa: E402
+
+import myers_amd as M  # noqa: E402
+
+# MI355X published figures: 256 CUs, 2.4 GHz peak engine clock, 8 TB/s HBM3E.
+CUS, CLOCK_HZ, LANES_PER_CU_CLK = 256, 2.4e9, 64
+PEAK_INT_OPS = CUS * LANES_PER_CU_CLK * CLOCK_HZ            # 3.93e13 int32 lane-ops/s
+PEAK_HBM_GBS = 8000.0
+# Algorithmic VALU ops per 32 cells (one u32 word of one column):
+#   K_GFX950 = 10: and, add-with-carry, 3 three-input logic ops (Xh, Ph, Pv),
+#   Mh and, Mv and, Xv or, 2 funnel shifts -- the recurrence's minimum on gfx950.
+#   K_REF = 18: the reference's u32 formulation (src/myers-4bits.c:117-134).
+K_GFX950 = 10
+K_REF = 18
+
+CONFIGS = {
+    1: dict(name="config1", ref_seed=42, ref_size=4_000_000, reads=10_000, spec=dict(
+        length=100, sub=0.02, ins=0.0025, dele=0.0025, candidates=11)),
+    2: dict(name="config2", ref_seed=7, ref_size=3_100_000_000, reads=100_000, spec=dict(
+        length=1000, sub=0.03, ins=0.005, dele=0.005, candidates=11)),
+    3: dict(name="config3", ref_seed=7, ref_size=3_100_000_000, reads=1_000_000, spec=dict(
+        choices=[100, 200, 400, 600, 800, 1000], err_range=(0.01, 0.05), sub_frac=0.8, poisson=11, cap=64)),
+    4: dict(name="config4", ref_seed=7, ref_size=3_100_000_000, reads=50_000, spec=dict(
+        log_uniform=(2000, 20000), err_range=(0.12, 0.12), sub_frac=0.5, candidates=4)),
+}
+DISTANCE = 0.1
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def window_len(m):
+    """n = (uint32)(m * (1 + 2*d)) in float32 (src-api/coreMyersGPU-CC35.cu:131)."""
+    f = np.float32(1.0) + np.float32(2.0) * np.float32(DISTANCE)
+    return np.floor(np.asarray(m).astype(np.float32) * f).astype(np.int64)
+
+
+def algorithmic_bytes(qinfo, cands):
+    """Text window (n/2 B packed) + candidate (16 B) + result (8 B) per
+    candidate, plus Peq (80 B per 128 bases) and qinfo (8 B) per read."""
+    m = qinfo[cands["query"], 1].astype(np.int64)
+    n = window_len(m)
+    per_cand = (n + 1) // 2 + 16 + 8
+    per_read = 80 * ((qinfo[:, 1].astype(np.int64) + 127) // 128) + 8
+    return int(per_cand.sum() + per_read.sum())
+
+
+def cpu_baseline(ref, ref_size, b, gpu_results, seconds):
+    """Time the oracle (C restatement of src/myers-4bits*.c, OpenMP) on a
+    seeded sample of the same candidates and check the GPU results on it."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib as O
+    rng = np.random.default_rng(1234)
+    cands = b["cands"]
+    order = rng.permutation(len(cands))
+    n = 256
+    elapsed = 0.0
+    checked = 0
+    while True:
+        idx = np.sort(order[:n])
+        t0 = time.perf_counter()
+        res = O.myers_batch(ref, ref_size, b["peq"], b["qinfo"], cands[idx], DISTANCE, O.MODE_ALIGNED)
+        elapsed = time.perf_counter() - t0
+        mismatches = int((res != gpu_results[idx]).any(axis=1).sum())
+        checked = n
+        if mismatches:
+            raise SystemExit(f"bench: GPU results differ from the oracle on {mismatches} of {n} sampled candidates")
+        if elapsed >= seconds * 0.5 or n >= len(cands):
+            break
+        n = min(len(cands), int(n * max(2.0, min(16.0, seconds / max(elapsed, 1e-3)))))
+    m = b["qinfo"][cands["query"][idx], 1].astype(np.int64)
+    cells = int((m * window_len(m)).sum())
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": checked / elapsed,
+        "unit": "candidates/s",
+        "gcups": cells / elapsed / 1e9,
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{checked} seeded-random candidates of the same batch ({elapsed:.1f} s, OpenMP static); "
+                  f"GPU results bit-exact on all of them",
+        "cpu_model": _cpu_model(),
+    }
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
+    ap.add_argument("--engine", default="auto", choices=["auto", "plain", "persistent"])
+    ap.add_argument("--mode", default="aligned", choices=["aligned", "exact"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (candidates shard by read "
+              "across GPUs with a replicated reference; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    cfg = CONFIGS[args.config]
+    t0 = time.time()
+    ref = M.synth_reference(cfg["ref_seed"], cfg["ref_size"])
+    log(f"reference {cfg['ref_size']} bases in {time.time() - t0:.1f}s")
+    spec = M.make_spec(cfg["ref_seed"] + 1, cfg["reads"], distance=DISTANCE, **cfg["spec"])
+    t0 = time.time()
+    b = M.synth_batch(spec, ref, cfg["ref_size"])
+    nc, nq = len(b["cands"]), len(b["qinfo"])
+    log(f"batch: {nq} reads, {nc} candidates, {b['cells']:.3e} cells in {time.time() - t0:.1f}s")
+
+    flags = (M.WINDOW_EXACT if args.mode == "exact" else M.WINDOW_ALIGNED) | \
+        {"auto": M.ENGINE_AUTO, "plain": M.ENGINE_PLAIN, "persistent": M.ENGINE_PERSISTENT}[args.engine]
+    avg = int(b["qinfo"][:, 1].mean())
+    m = M.Myers(ref, cfg["ref_size"], avg_query=avg, cands_per_query=max(1, nc // nq),
+                buffer_bytes=max(256 << 20, 2 * (b["peq"].nbytes + b["cands"].nbytes + nc * 12)))
+    dp, dq, dc = M.DeviceArray(b["peq"]), M.DeviceArray(b["qinfo"]), M.DeviceArray(b["cands"])
+    dr = M.DeviceArray.empty((nc, 2), np.uint32)
+
+    def step():
+        M.verify_device(m, dp, len(b["peq"]), dq, nq, dc, nc, DISTANCE, dr, flags=flags, h_qinfo=b["qinfo"])
+
+    for _ in range(args.warmup):
+        step()
+    M.synchronize(m)
+    M.lib().myersResetStats(m.buffer(0))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    M.synchronize(m)     # each step's completion is waited on by the next synchronize
+    t1 = time.perf_counter()
+    st = m.stats()
+    # The library records HIP events on the buffer's stream around each
+    # step's verification launches; myersSynchronize folds them in.
+    elapsed = t1 - t0
+    results = dr.get()
+    if st.batchesTotal != args.steps:
+        raise SystemExit(f"bench: {st.batchesTotal} timed batches for {args.steps} steps")
+    kernel_ms = st.kernelMsTotal / st.batchesTotal     # mean kernel time per step
+    value = nc * args.steps / elapsed
+    cells = b["cells"]
+    gcups = cells * args.steps / elapsed / 1e9
+    kernel_s = kernel_ms / 1e3
+    valu_achieved = cells * K_GFX950 / 32.0 / kernel_s / 1e12
+    hbm_bytes = algorithmic_bytes(b["qinfo"], b["cands"])
+    hbm_achieved = hbm_bytes / kernel_s / 1e9
+
+    info = m.variants()
+    used = sorted({int(v) for v in m.bin_table()[np.unique((b["qinfo"][:, 1] + 31) // 32)]})
+    variants = [f"L{info[v]['lanes']}xW{info[v]['words']}" for v in used]
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        log("CPU baseline / parity sample ...")
+        cpu = cpu_baseline(ref, cfg["ref_size"], b, results, args.cpu_seconds)
+
+    out = {
+        "metric": "candidates/s & GCUPS on 1 MI355X (bit-exact); p99 latency under serving load",
+        "value": value,
+        "unit": "candidates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded generator, see include/myers-tools.h)",
+        "gcups": gcups,
+        "config": {
+            "workload": f"{cfg['name']}: seed-{cfg['ref_seed']} {cfg['ref_size'] / 1e9:.3g} Gbase reference, "
+                        f"{nq} reads, {nc} candidates, window = len*1.2, {args.mode} windows",
+            "reads": nq, "candidates": nc, "cells": cells, "reference_bases": cfg["ref_size"],
+            "engine": args.engine, "variants": variants, "launches_per_step": st.launches,
+            "binned": bool(st.binned),
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": valu_achieved,
+            "peak": PEAK_INT_OPS / 1e12,
+            "unit": "TOP/s",
+            "frac": valu_achieved / (PEAK_INT_OPS / 1e12),
+            "traffic": None,
+            "kernel_ms": kernel_ms,
+            "kernel_launches_per_step": st.kernelLaunchesTotal / st.batchesTotal,
+            "ops_per_cell": K_GFX950 / 32.0,
+            "frac_reference_ops": cells * K_REF / 32.0 / kernel_s / PEAK_INT_OPS,
+            "hbm": {"achieved": hbm_achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": hbm_achieved / PEAK_HBM_GBS, "algorithmic_bytes": hbm_bytes},
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    for a in (dp, dq, dc, dr):
+        a.free()
+    m.close()
+
+
+if __name__ == "__main__":
+    main()

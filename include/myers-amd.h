@@ -69,14 +69,15 @@ typedef struct {
     uint32_t launches;          /* verification kernel launches in the last batch */
     uint32_t variantsUsed;      /* distinct kernel variants in the last batch */
     uint32_t binned;            /* 1 if the batch went through device-side binning */
-    double   kernelMs;          /* HIP-event time of the verification kernels, last batch */
+    double   kernelMs;          /* HIP-event time of the verification kernels, last timed batch */
     double   batchMs;           /* HIP-event time from first copy to last copy */
     uint64_t cells;             /* sum of m * n over verified candidates (nominal window) */
     uint64_t candidates;
-    /* accumulated over the buffer's life (or since myersResetStats) */
-    double   kernelMsTotal;
-    uint64_t kernelLaunchesTotal;
-    uint64_t batchesTotal;
+    /* accumulated over the buffer's life (or since myersResetStats), folded
+     * in when the stream is synchronised (receive / myersSynchronize) */
+    double   kernelMsTotal;         /* sum over batches of their kernel time */
+    uint64_t kernelLaunchesTotal;   /* verification launches */
+    uint64_t batchesTotal;          /* timed batches */
 } myersStats_t;
 
 void myersDefaultConfig(myersConfig_t *cfg);

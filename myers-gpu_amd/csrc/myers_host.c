@@ -37,6 +37,7 @@
 #define MAX_QUERY_WORDS  (MYERS_MAX_QUERY_SIZE / 32u)
 #define STAGE_BYTES      (64u << 20)    /* pinned staging for the reference upload */
 #define DEFAULT_BUFFER_CAP (2ull << 30)
+#define EV_RING          32u            /* kernel-timing event pairs per buffer */
 
 typedef struct myers_ctx {
     int       device;
@@ -71,7 +72,13 @@ typedef struct myers_buffer {
     uint32_t    *d_order;
     uint32_t    *d_work;          /* counts | cursors | ranges(2x) | err */
     uint32_t    *h_err;           /* pinned copy of the binning error flag */
-    hipEvent_t   ev_batch_start, ev_batch_end, ev_kernel_start, ev_kernel_end;
+    hipEvent_t   ev_batch_start, ev_batch_end;
+    /* HIP events around the verification launches of each batch, in a ring
+     * so that batches enqueued back to back are all timed; folded into the
+     * stats in order when the stream is synchronised. */
+    hipEvent_t   ev_kernel[EV_RING][2];
+    uint32_t     ev_launches[EV_RING];
+    uint32_t     ev_head, ev_pending;
     int          exit_on_error;
     int          last_error;
     int          pending;         /* a batch was enqueued and not yet received */
@@ -247,8 +254,10 @@ static void free_buffer(myers_buffer_t *b)
     if (b->d_work) hipFree(b->d_work);
     if (b->ev_batch_start) hipEventDestroy(b->ev_batch_start);
     if (b->ev_batch_end) hipEventDestroy(b->ev_batch_end);
-    if (b->ev_kernel_start) hipEventDestroy(b->ev_kernel_start);
-    if (b->ev_kernel_end) hipEventDestroy(b->ev_kernel_end);
+    for (uint32_t i = 0; i < EV_RING; i++) {
+        if (b->ev_kernel[i][0]) hipEventDestroy(b->ev_kernel[i][0]);
+        if (b->ev_kernel[i][1]) hipEventDestroy(b->ev_kernel[i][1]);
+    }
     if (b->stream) hipStreamDestroy(b->stream);
     free(b);
 }
@@ -273,8 +282,10 @@ static int init_buffer(myers_buffer_t *b)
     HIP_TRY(hipMalloc((void **)&b->d_work, (5u * MYERS_MAX_VARIANTS + 1u) * sizeof(uint32_t)));
     HIP_TRY(hipEventCreate(&b->ev_batch_start));
     HIP_TRY(hipEventCreate(&b->ev_batch_end));
-    HIP_TRY(hipEventCreate(&b->ev_kernel_start));
-    HIP_TRY(hipEventCreate(&b->ev_kernel_end));
+    for (uint32_t i = 0; i < EV_RING; i++) {
+        HIP_TRY(hipEventCreate(&b->ev_kernel[i][0]));
+        HIP_TRY(hipEventCreate(&b->ev_kernel[i][1]));
+    }
     memset(b->h_results, 0, re_b);
     *b->h_err = 0;
     b->stats.structSize = sizeof(myersStats_t);
@@ -484,6 +495,43 @@ static uint32_t grid_for(const myers_ctx_t *ctx, uint32_t v, uint32_t count, int
     return (uint32_t)chunks;
 }
 
+/* Fold the oldest timed batch into the stats (waits for its end event). */
+static int fold_oldest_timing(myers_buffer_t *b)
+{
+    const uint32_t slot = (b->ev_head + EV_RING - b->ev_pending) % EV_RING;
+    int rc = hip_fail(hipEventSynchronize(b->ev_kernel[slot][1]), "hipEventSynchronize", __FILE__, __LINE__);
+    if (rc) return rc;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, b->ev_kernel[slot][0], b->ev_kernel[slot][1]) == hipSuccess) {
+        b->stats.kernelMs = ms;
+        b->stats.kernelMsTotal += ms;
+        b->stats.kernelLaunchesTotal += b->ev_launches[slot];
+        b->stats.batchesTotal++;
+    }
+    b->ev_pending--;
+    return MYERS_SUCCESS;
+}
+
+static int timing_begin(myers_buffer_t *b, uint32_t *slot)
+{
+    if (b->ev_pending == EV_RING) {
+        int rc = fold_oldest_timing(b);
+        if (rc) return rc;
+    }
+    *slot = b->ev_head;
+    return hip_fail(hipEventRecord(b->ev_kernel[*slot][0], b->stream), "hipEventRecord", __FILE__, __LINE__);
+}
+
+static int timing_end(myers_buffer_t *b, uint32_t slot, uint32_t launches)
+{
+    int rc = hip_fail(hipEventRecord(b->ev_kernel[slot][1], b->stream), "hipEventRecord", __FILE__, __LINE__);
+    if (rc) return rc;
+    b->ev_launches[slot] = launches;
+    b->ev_head = (slot + 1u) % EV_RING;
+    b->ev_pending++;
+    return MYERS_SUCCESS;
+}
+
 /* Enqueue the verification of num_cands candidates on b->stream. */
 static int enqueue_verify(myers_buffer_t *b, const void *d_peq, uint32_t num_entries, const void *d_qinfo,
                           const void *d_cands, uint32_t num_cands, float distance, uint32_t flags,
@@ -517,31 +565,28 @@ static int enqueue_verify(myers_buffer_t *b, const void *d_peq, uint32_t num_ent
     b->stats.launches = 0;
     b->stats.variantsUsed = 0;
     b->binned = 0;
-    if (num_cands == 0) {
-        HIP_TRY(hipEventRecord(b->ev_kernel_start, b->stream));
-        HIP_TRY(hipEventRecord(b->ev_kernel_end, b->stream));
-        return MYERS_SUCCESS;
-    }
+    uint32_t slot = 0;
+    int rc;
+    if (num_cands == 0) return MYERS_SUCCESS;
     if (scan.num_present == 1 && !scan.too_long) {
         const uint32_t v = scan.single;
         a.order = NULL;
         a.bin_ranges = NULL;
         a.start = 0;
         a.count = num_cands;
-        HIP_TRY(hipEventRecord(b->ev_kernel_start, b->stream));
-        int rc = myers_launch_verify(v, &a, grid_for(ctx, v, num_cands, 0), b->stream);
+        if ((rc = timing_begin(b, &slot))) return rc;
+        rc = myers_launch_verify(v, &a, grid_for(ctx, v, num_cands, 0), b->stream);
         if (rc) return hip_fail((hipError_t)rc, "verify launch", __FILE__, __LINE__);
-        HIP_TRY(hipEventRecord(b->ev_kernel_end, b->stream));
         b->stats.launches = 1;
         b->stats.variantsUsed = 1;
-        return MYERS_SUCCESS;
+        return timing_end(b, slot, 1u);
     }
     /* device-side binning */
     uint32_t *counts = b->d_work;
     uint32_t *cursors = counts + MYERS_MAX_VARIANTS;
     uint32_t *ranges = cursors + MYERS_MAX_VARIANTS;
     uint32_t *err = ranges + 2u * MYERS_MAX_VARIANTS;
-    int rc = myers_launch_binning((const uint32_t *)d_qinfo, num_queries, d_cands, num_cands, ctx->d_bin_of_words,
+    rc = myers_launch_binning((const uint32_t *)d_qinfo, num_queries, d_cands, num_cands, ctx->d_bin_of_words,
                                   max_supported_words(ctx), ctx->num_variants, counts, cursors, ranges,
                                   b->d_order, err, d_results, b->stream);
     if (rc) return hip_fail((hipError_t)rc, "binning launch", __FILE__, __LINE__);
@@ -549,7 +594,7 @@ static int enqueue_verify(myers_buffer_t *b, const void *d_peq, uint32_t num_ent
     b->binned = 1;
     a.order = b->d_order;
     a.bin_ranges = ranges;
-    HIP_TRY(hipEventRecord(b->ev_kernel_start, b->stream));
+    if ((rc = timing_begin(b, &slot))) return rc;
     /* longest queries first: they have the fewest workgroups per launch */
     for (int v = (int)ctx->num_variants - 1; v >= 0; v--) {
         if (!(scan.present & (1ull << v))) continue;
@@ -559,8 +604,7 @@ static int enqueue_verify(myers_buffer_t *b, const void *d_peq, uint32_t num_ent
         b->stats.launches++;
         b->stats.variantsUsed++;
     }
-    HIP_TRY(hipEventRecord(b->ev_kernel_end, b->stream));
-    return MYERS_SUCCESS;
+    return timing_end(b, slot, b->stats.launches);
 }
 
 static uint64_t nominal_cells(const qryInfo_t *qinfo, const candInfo_t *cands, uint32_t nc, float d)
@@ -616,15 +660,13 @@ static int finish_batch(myers_buffer_t *b, int with_batch_events)
 {
     int rc = hip_fail(hipStreamSynchronize(b->stream), "hipStreamSynchronize", __FILE__, __LINE__);
     if (rc) return rc;
+    while (b->ev_pending) {
+        rc = fold_oldest_timing(b);
+        if (rc) return rc;
+    }
     if (!b->pending) return MYERS_SUCCESS;
     b->pending = 0;
     float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, b->ev_kernel_start, b->ev_kernel_end) == hipSuccess) {
-        b->stats.kernelMs = ms;
-        b->stats.kernelMsTotal += ms;
-    }
-    b->stats.kernelLaunchesTotal += b->stats.launches;
-    b->stats.batchesTotal++;
     if (with_batch_events && hipEventElapsedTime(&ms, b->ev_batch_start, b->ev_batch_end) == hipSuccess)
         b->stats.batchMs = ms;
     b->stats.binned = (uint32_t)b->binned;

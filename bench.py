@@ -48,6 +48,10 @@ CONFIGS = {
         log_uniform=(2000, 20000), err_range=(0.12, 0.12), sub_frac=0.5, candidates=4)),
 }
 DISTANCE = 0.1
+# HBM bytes per launch of the default workload (config 2, aligned windows),
+# from FETCH_SIZE and WRITE_SIZE counter runs scaled by a 1 GiB copy
+# calibration (profiles/hbm_traffic_verify_L1W32_config2.md).
+MEASURED_TRAFFIC = {(2, "aligned"): 920.4e6}
 
 
 def log(msg):
@@ -127,10 +131,11 @@ def main():
     ap.add_argument("--engine", default="auto", choices=["auto", "plain", "persistent"])
     ap.add_argument("--mode", default="aligned", choices=["aligned", "exact"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
+    ap.add_argument("--variant", default=None, help="force a kernel variant for every read length, e.g. L2xW16")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (candidates shard by read "
-              "across GPUs with a replicated reference; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (candidates would shard by read "
+              "across GPUs that each hold a copy of the reference). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     cfg = CONFIGS[args.config]
@@ -148,6 +153,16 @@ def main():
     avg = int(b["qinfo"][:, 1].mean())
     m = M.Myers(ref, cfg["ref_size"], avg_query=avg, cands_per_query=max(1, nc // nq),
                 buffer_bytes=max(256 << 20, 2 * (b["peq"].nbytes + b["cands"].nbytes + nc * 12)))
+    if args.variant:
+        lanes, words = (int(x) for x in args.variant.lstrip("L").split("xW"))
+        info = m.variants()
+        v = [i for i, d in enumerate(info) if d["lanes"] == lanes and d["words"] == words]
+        if not v:
+            raise SystemExit(f"bench: no variant {args.variant}")
+        table = m.bin_table()
+        w = np.arange(len(table))
+        table[(w >= 1) & (w <= lanes * words)] = v[0]
+        m.set_bin_table(table)
     dp, dq, dc = M.DeviceArray(b["peq"]), M.DeviceArray(b["qinfo"]), M.DeviceArray(b["cands"])
     dr = M.DeviceArray.empty((nc, 2), np.uint32)
 
@@ -215,7 +230,8 @@ def main():
             "peak": PEAK_INT_OPS / 1e12,
             "unit": "TOP/s",
             "frac": valu_achieved / (PEAK_INT_OPS / 1e12),
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get((args.config, args.mode)),
+            "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "kernel_ms": kernel_ms,
             "kernel_launches_per_step": st.kernelLaunchesTotal / st.batchesTotal,
             "ops_per_cell": K_GFX950 / 32.0,

@@ -154,10 +154,14 @@ void myersDefaultConfig(myersConfig_t *cfg)
 
 static uint32_t variant_cost(const myers_variant_desc_t *v)
 {
-    /* VALU work per candidate column is ~10 ops per u32 word; lanes > 1 add
-     * a cross-lane carry/shift pass that costs roughly a fifth more. */
-    uint32_t base = v->total_words * 10u + 12u * v->lanes;
-    return v->lanes > 1u ? base + base / 5u : base;
+    /* Estimated issue slots per candidate column: ~10 VALU ops per u32 word
+     * plus ~12 per lane (text, score, minimum) and ~6 per lane for the
+     * diagonal hand-over when L > 1.  Variants whose LDS leaves room for
+     * fewer than two waves per SIMD (over 20 KB per one-wave workgroup)
+     * cannot use the dual-issue rate of and/or/bitop3 and pay ~40%. */
+    uint32_t cost = v->total_words * 10u + 12u * v->lanes + (v->lanes > 1u ? 6u * v->lanes : 0u);
+    if (v->lds_bytes > 20u * 1024u) cost += (cost * 2u) / 5u;
+    return cost;
 }
 
 /* Default length-bin table: for w query words pick the variant that covers

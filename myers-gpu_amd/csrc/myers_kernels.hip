@@ -2,24 +2,39 @@
 //
 // Each candidate is verified by the Myers/Hyyro bit-vector recurrence over
 // its reference window (reference kernel: src-api/coreMyersGPU-CC35.cu:110-240;
-// CPU form: src/myers-4bits.c:111-135).  Design (all integer VALU work, no
-// MFMA):
+// CPU form: src/myers-4bits.c:111-135).  All work is integer VALU (no MFMA).
+//
 //   * A kernel variant <L, W> gives each candidate L lanes of a wave64 and
 //     each lane W u32 words of the query bit-vector (32*L*W rows in all).
-//     L = 1 keeps the whole recurrence inside one lane (no cross-lane work);
-//     L > 1 splits long queries across lanes with a carry-lookahead over two
-//     wave ballots for the addition and a ballot shift for the Ph/Mh bits.
+//     Inside a lane the W words form one full-width vector (add carries and
+//     Ph/Mh shifts run through all of them).  Between the lanes of a group
+//     the block form of the recurrence applies (src/myers-4bits.c:115-134):
+//     a lane needs only the horizontal delta hin at its bottom boundary,
+//     which is the top bit of the Ph/Mh of the lane below in the same
+//     column.  The lanes run diagonally: lane b of a group computes column
+//     t - b at step t, so its hin is the lower lane's output of step t - 1,
+//     moved up one lane by a single DPP shift.  No carry crosses lanes, no
+//     ballot, no branch.  A lane before its first column reads an all-zero
+//     Peq row; with Eq = 0 and hin = 0 the initial state Pv = ~0, Mv = 0 is
+//     a fixed point emitting no delta, so late-starting lanes need no reset.
+//     L = 1 is the special case with hin = 0 and no lag.
 //   * Query bit-vectors are top-justified: a query of m bases occupies the
 //     top m rows and the rows below it never match (their Peq bits are 0).
-//     Such rows stay at the fixed point Pv=~0, Mv=0, so they emit no
-//     horizontal delta and the query rows see the semi-global boundary
-//     D[0][j]=0 unchanged.  The score row m-1 is therefore always bit 31 of
-//     the top lane's top word, whatever m is within the variant.
-//   * Peq masks are transposed into LDS per lane as [W/4][5 bases][64 lanes]
-//     x 128-bit, so one column's Eq words are ceil(W/4) conflict-free
+//     Such rows stay at the same fixed point, so they emit no horizontal
+//     delta and the query rows see the semi-global boundary D[0][j]=0.  The
+//     score row m-1 is therefore bit 31 of the top lane's top word for every
+//     m the variant covers.
+//   * Peq masks are transposed into LDS per lane as
+//     [ceil(W/4)][rows][64 lanes] x 128-bit (rows = 5 bases, plus a zero row
+//     for L > 1), so one column's Eq words are ceil(W/4) conflict-free
 //     ds_read_b128 at an address chosen by the column's base code.
-//   * The packed reference (8 bases per u32) streams through a two-word
-//     register window per lane; exact-start windows use a funnel shift.
+//   * The packed reference (8 bases per u32) streams through a register
+//     window per lane, two 8-column blocks ahead of use; exact-start windows
+//     and the diagonal lag use a per-lane funnel shift.
+//   * Per word and column the recurrence is 10 VALU ops: and, add-with-carry,
+//     3 three-input logic ops, 2 funnel shifts, or, and, and.  and/or/bitop3
+//     issue at twice the rate of the funnel shifts and the carry add on
+//     gfx950, so variants are sized for at least two waves per SIMD.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -34,6 +49,7 @@ struct CandRec {
 };
 
 constexpr uint32_t kSkipScore = 0xFFFFFFFFu;
+constexpr uint32_t kZeroRow = 5u;          // LDS row of all-zero Peq words (L > 1)
 
 // n = (uint32)(m * (1 + 2*d)) in float32 with round-to-nearest steps
 // (src-api/coreMyersGPU-CC35.cu:131).
@@ -58,27 +74,13 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
-template <int L>
-struct LaneMasks {
-    // bit i set iff lane i is the top (resp. first) lane of its group
-    static constexpr uint64_t top() {
-        uint64_t m = 0;
-        for (int i = L - 1; i < 64; i += L) m |= 1ull << i;
-        return m;
-    }
-    static constexpr uint64_t first() {
-        uint64_t m = 0;
-        for (int i = 0; i < 64; i += L) m |= 1ull << i;
-        return m;
-    }
-};
-
 template <int L, int W>
 struct Shape {
     static constexpr int kGroups = (W + 3) / 4;        // 128-bit LDS groups per lane
     static constexpr int kTotal = L * W;                // words per candidate
     static constexpr int kSlots = 64 / L;               // candidates per wave
-    static constexpr int kLdsU4 = kGroups * 5 * 64;
+    static constexpr int kRows = L > 1 ? 6 : 5;         // A C G T N (+ zero row)
+    static constexpr int kLdsU4 = kGroups * kRows * 64;
 };
 
 // Stage the Peq words of this lane into LDS, top-justified.  Lane word k
@@ -111,10 +113,14 @@ __device__ __forceinline__ void stage_peq(uint4 *lds, const uint32_t *__restrict
             uint32_t next = 0;
             if (j >= 0 && j < nq) next = base[(uint32_t)(j >> 2) * 20u + (uint32_t)(j & 3) + 4u * c];
             const uint32_t v = (uint32_t)((((uint64_t)next << 32) | prev) >> sh);
-            l32[(((k >> 2) * 5 + c) * 64 + lane) * 4 + (k & 3)] = v;
+            l32[(((k >> 2) * S::kRows + c) * 64 + lane) * 4 + (k & 3)] = v;
             prev = next;
         }
         __builtin_amdgcn_sched_barrier(0);
+    }
+    if constexpr (S::kRows > 5) {
+#pragma unroll
+        for (int k = 0; k < W; k++) l32[(((k >> 2) * S::kRows + kZeroRow) * 64 + lane) * 4 + (k & 3)] = 0u;
     }
 }
 
@@ -124,7 +130,7 @@ __device__ __forceinline__ void load_eq(const uint4 *lds_lane, uint32_t row_off,
     const uint4 *p = lds_lane + row_off;
 #pragma unroll
     for (int g = 0; g < S::kGroups; g++) {
-        uint4 v = p[g * 5 * 64];
+        uint4 v = p[g * S::kRows * 64];
         eq[4 * g] = v.x;
         if (4 * g + 1 < W) eq[4 * g + 1] = v.y;
         if (4 * g + 2 < W) eq[4 * g + 2] = v.z;
@@ -138,22 +144,23 @@ __device__ __forceinline__ void load_eq(const uint4 *lds_lane, uint32_t row_off,
 constexpr uint32_t kXorOr = (0xF0u ^ 0xCCu) | 0xAAu;          // (a ^ b) | c
 constexpr uint32_t kOrNor = (0xF0u | ~(0xCCu | 0xAAu)) & 0xFFu;  // a | ~(b | c)
 
-// One column of the recurrence for a lane-local bit-vector (L = 1):
-//   Xv = Eq|Mv;  Xh = (((Eq&Pv)+Pv)^Pv)|Eq;  Ph = Mv|~(Xh|Pv);  Mh = Pv&Xh
-//   Ph <<= 1; Mh <<= 1 (row 0 gets 0: D[0][j] = 0);  Pv = Mh|~(Xv|Ph);  Mv = Ph&Xv
-// The carry of the addition and the bits shifted out of each word run
-// through the whole 32*W-bit vector: 10 VALU ops per word (and, add-carry,
-// 3 bitop3, 2 funnel shifts, or, bitop3, and).  Returns the top word's
-// Ph/Mh (before the shift) for the score row.
-template <int W>
-__device__ __forceinline__ void column_local(uint32_t (&Pv)[W], uint32_t (&Mv)[W],
-                                             const uint32_t (&Eq)[W], uint32_t &ph_top,
-                                             uint32_t &mh_top) {
-    uint32_t carry = 0, ph_prev = 0, mh_prev = 0;
+// One column for one lane's W words (src/myers-4bits.c:117-134 with a
+// 32W-bit block):
+//   Xv = Eq|Mv;  Eq |= [hin = -1];  Xh = (((Eq&Pv)+Pv)^Pv)|Eq
+//   Ph = Mv|~(Xh|Pv);  Mh = Pv&Xh;  hout = top bits of Ph, Mh
+//   Ph = Ph<<1 | [hin = +1];  Mh = Mh<<1 | [hin = -1];  Pv = Mh|~(Xv|Ph);  Mv = Ph&Xv
+// ph_in / mh_in carry hin in bit 31 (the lower lane's top Ph / Mh word; 0
+// for the bottom lane: D[0][j] = 0).
+template <int W, bool kHin>
+__device__ __forceinline__ void column_step(uint32_t (&Pv)[W], uint32_t (&Mv)[W], const uint32_t (&Eq)[W],
+                                            uint32_t ph_in, uint32_t mh_in, uint32_t &ph_top,
+                                            uint32_t &mh_top) {
+    uint32_t carry = 0, ph_prev = ph_in, mh_prev = mh_in;
 #pragma unroll
     for (int k = 0; k < W; k++) {
-        const uint32_t pv = Pv[k], mv = Mv[k], eq = Eq[k];
-        const uint32_t xv = eq | mv;
+        const uint32_t pv = Pv[k], mv = Mv[k];
+        const uint32_t xv = Eq[k] | mv;
+        const uint32_t eq = (kHin && k == 0) ? (Eq[0] | (mh_in >> 31)) : Eq[k];
         uint32_t c_out;
         const uint32_t sum = __builtin_addc(eq & pv, pv, carry, &c_out);
         carry = c_out;
@@ -164,8 +171,8 @@ __device__ __forceinline__ void column_local(uint32_t (&Pv)[W], uint32_t (&Mv)[W
             ph_top = ph;
             mh_top = mh;
         }
-        const uint32_t php = k == 0 ? (ph << 1) : __builtin_amdgcn_alignbit(ph, ph_prev, 31u);
-        const uint32_t mhp = k == 0 ? (mh << 1) : __builtin_amdgcn_alignbit(mh, mh_prev, 31u);
+        const uint32_t php = (!kHin && k == 0) ? (ph << 1) : __builtin_amdgcn_alignbit(ph, ph_prev, 31u);
+        const uint32_t mhp = (!kHin && k == 0) ? (mh << 1) : __builtin_amdgcn_alignbit(mh, mh_prev, 31u);
         ph_prev = ph;
         mh_prev = mh;
         Pv[k] = __builtin_amdgcn_bitop3_b32(mhp, xv, php, kOrNor);
@@ -173,75 +180,17 @@ __device__ __forceinline__ void column_local(uint32_t (&Pv)[W], uint32_t (&Mv)[W
     }
 }
 
-// One column for a bit-vector split over L lanes (W words each).  Each lane
-// adds its words with carry-in 0; the carry into every lane then follows
-// from G (lane carry-out) and P (lane sum all ones) by one 64-bit add:
-// carries = (G + (G|P)) ^ G ^ (G|P), with G and P cleared at group tops so
-// no carry crosses into the next candidate.  The Ph/Mh bit shifted out of a
-// lane's top word enters bit 0 of the next lane via a ballot shifted by one.
-template <int L, int W>
-__device__ __forceinline__ void column_group(uint32_t (&Pv)[W], uint32_t (&Mv)[W],
-                                             const uint32_t (&Eq)[W], uint32_t &ph_top,
-                                             uint32_t &mh_top) {
-    constexpr uint64_t kTop = LaneMasks<L>::top();
-    constexpr uint64_t kFirst = LaneMasks<L>::first();
-    uint32_t sum[W];
-    uint32_t carry = 0, all_ones = ~0u;
-#pragma unroll
-    for (int k = 0; k < W; k++) {
-        uint32_t c_out;
-        sum[k] = __builtin_addc(Eq[k] & Pv[k], Pv[k], carry, &c_out);
-        carry = c_out;
-        all_ones &= sum[k];
-    }
-    const uint64_t G = __ballot(carry != 0u) & ~kTop;
-    const uint64_t P = __ballot(all_ones == ~0u) & ~kTop;
-    const uint64_t X = G | P;
-    const uint64_t cin = (G + X) ^ G ^ X;
-    uint32_t c = __builtin_amdgcn_inverse_ballot_w64(cin) ? 1u : 0u;
-#pragma unroll
-    for (int k = 0; k < W; k++) {
-        uint32_t c_out;
-        sum[k] = __builtin_addc(sum[k], 0u, c, &c_out);
-        c = c_out;
-    }
-    uint32_t ph[W], mh[W];
-#pragma unroll
-    for (int k = 0; k < W; k++) {
-        const uint32_t xh = __builtin_amdgcn_bitop3_b32(sum[k], Pv[k], Eq[k], kXorOr);
-        ph[k] = __builtin_amdgcn_bitop3_b32(Mv[k], xh, Pv[k], kOrNor);
-        mh[k] = Pv[k] & xh;
-    }
-    ph_top = ph[W - 1];
-    mh_top = mh[W - 1];
-    const uint64_t bp = (__ballot((ph[W - 1] >> 31) != 0u) << 1) & ~kFirst;
-    const uint64_t bm = (__ballot((mh[W - 1] >> 31) != 0u) << 1) & ~kFirst;
-    const uint32_t ph_in = __builtin_amdgcn_inverse_ballot_w64(bp) ? 1u : 0u;
-    const uint32_t mh_in = __builtin_amdgcn_inverse_ballot_w64(bm) ? 1u : 0u;
-#pragma unroll
-    for (int k = 0; k < W; k++) {
-        const uint32_t xv = Eq[k] | Mv[k];
-        const uint32_t php = k == 0 ? ((ph[0] << 1) | ph_in) : __builtin_amdgcn_alignbit(ph[k], ph[k - 1], 31u);
-        const uint32_t mhp = k == 0 ? ((mh[0] << 1) | mh_in) : __builtin_amdgcn_alignbit(mh[k], mh[k - 1], 31u);
-        Pv[k] = __builtin_amdgcn_bitop3_b32(mhp, xv, php, kOrNor);
-        Mv[k] = php & xv;
-    }
-}
-
-template <int L, int W>
-__device__ __forceinline__ void run_column(const uint32_t (&eq)[W], uint32_t col, uint32_t ncols,
-                                           uint32_t (&Pv)[W], uint32_t (&Mv)[W], int32_t &score,
-                                           int32_t &min_score, uint32_t &min_col) {
-    uint32_t ph_top, mh_top;
-    if constexpr (L == 1) {
-        column_local<W>(Pv, Mv, eq, ph_top, mh_top);
+// The lower lane's value of the previous step: DPP row_shr:1 (within rows
+// of 16) or wave_shr:1, with 0 on the first lane of every group.
+template <int L>
+__device__ __forceinline__ uint32_t from_lower_lane(uint32_t v, uint32_t not_first) {
+    if constexpr (L > 16) {
+        v = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, true);   // wave_shr:1
     } else {
-        column_group<L, W>(Pv, Mv, eq, ph_top, mh_top);
+        v = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true);   // row_shr:1
     }
-    score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
-    const bool better = (score < min_score) && (col < ncols);
-    min_col = better ? col : min_col;
-    min_score = better ? score : min_score;
+    if constexpr (L != 16 && L != 64) v &= not_first;   // rows and the wave already start at 0
+    return v;
 }
 
 template <int L, int W>
@@ -250,6 +199,7 @@ __global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
     __shared__ uint4 lds[S::kLdsU4];
 
     const uint32_t lane = threadIdx.x;
+    const uint32_t gl = lane % L;                      // lane within the candidate's group
     uint32_t start = a.start, count = a.count;
     if (a.bin_ranges != nullptr) {
         start = a.bin_ranges[2 * a.variant];
@@ -257,11 +207,12 @@ __global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
     }
     const uint32_t nchunks = (count + S::kSlots - 1) / S::kSlots;
     const uint32_t *__restrict__ ref = a.ref;
-    const uint32_t last_word = (uint32_t)(a.ref_words - 1);
+    const int32_t last_word = (int32_t)(a.ref_words - 1);
     const CandRec *__restrict__ cands = static_cast<const CandRec *>(a.cands);
     const uint2 *__restrict__ qinfo = reinterpret_cast<const uint2 *>(a.qinfo);
     uint2 *__restrict__ results = static_cast<uint2 *>(a.results);
     const uint4 *lds_lane = lds + lane;
+    const uint32_t not_first = gl != 0u ? ~0u : 0u;
 
     // Workgroups are dispatched round-robin over the 8 XCDs; renumber them so
     // that each XCD (own L2) takes a contiguous run of chunks, which keeps the
@@ -307,7 +258,13 @@ __global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
         }
         stage_peq<L, W>(lds, a.peq, pos_entry, live ? m : 0u, lane);
 
+        // Steps: the top lane of a group finishes column ncols-1 at step
+        // ncols-1 + L-1.  Lane gl reads text from nibble nib0 - gl.
         const uint32_t max_cols = wave_max(ncols);
+        const uint32_t nblocks = (max_cols + (uint32_t)(L - 1) + 7u) >> 3;
+        const int64_t q = (int64_t)nib0 - (int64_t)gl;
+        const int32_t w0 = (int32_t)(q >> 3);            // floor
+        const uint32_t shift = (uint32_t)(q & 7) * 4u;
 
         uint32_t Pv[W], Mv[W];
 #pragma unroll
@@ -317,42 +274,66 @@ __global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
         }
         int32_t score = (int32_t)m, min_score = (int32_t)m;
         uint32_t min_col = 0;
+        uint32_t ph_top = 0, mh_top = 0;
 
         // Text words stream through registers two blocks (16 columns) ahead
-        // of their use; the Eq words of column j+1 are read from LDS while
-        // column j computes.  Reads past the stored reference are clamped
-        // to the last (all-N guard) word; they only feed columns that no
-        // lane of this wave keeps (col >= ncols).
-        const uint32_t w0 = (uint32_t)(nib0 >> 3);
-        const uint32_t shift = (uint32_t)(nib0 & 7u) * 4u;
-        const uint32_t tw0 = ref[min(w0, last_word)];
-        const uint32_t tw1 = ref[min(w0 + 1u, last_word)];
-        uint32_t tw2 = ref[min(w0 + 2u, last_word)];
-        uint32_t tw3 = ref[min(w0 + 3u, last_word)];
-        uint32_t text = sanitize_text(__builtin_amdgcn_alignbit(tw1, tw0, shift));
-        uint32_t text_next = sanitize_text(__builtin_amdgcn_alignbit(tw2, tw1, shift));
+        // of their use; the Eq words of step t+1 are read from LDS while
+        // step t computes.  Reads outside the stored reference are clamped
+        // to its first / last (all-N guard) word; they only feed steps that
+        // no lane keeps.
+        auto text_word = [&](int32_t k) { return ref[min(max(w0 + k, 0), last_word)]; };
+        auto text_block = [&](uint32_t lo, uint32_t hi, uint32_t b) {
+            uint32_t t = sanitize_text(__builtin_amdgcn_alignbit(hi, lo, shift));
+            if constexpr (L > 1) {
+                // steps before this lane's first column read the zero row
+                const int32_t early = (int32_t)gl - 8 * (int32_t)b;     // such steps in this block
+                if (early > 0) {
+                    const uint32_t low = early >= 8 ? ~0u : ((1u << (4 * early)) - 1u);
+                    t = (t & ~low) | (0x55555555u & low);
+                }
+            }
+            return t;
+        };
+        const uint32_t tw0 = text_word(0);
+        const uint32_t tw1 = text_word(1);
+        uint32_t tw2 = text_word(2);
+        uint32_t tw3 = text_word(3);
+        uint32_t text = text_block(tw0, tw1, 0);
+        uint32_t text_next = text_block(tw1, tw2, 1);
         uint32_t eq[W];
         load_eq<L, W>(lds_lane, (text & 7u) * 64u, eq);
-        const uint32_t nblocks = (max_cols + 7u) >> 3;
         for (uint32_t b = 0; b < nblocks; b++) {
-            const uint32_t tw4 = ref[min(w0 + b + 4u, last_word)];
+            const uint32_t tw4 = text_word((int32_t)b + 4);
 #pragma unroll
             for (int i = 0; i < 8; i++) {
                 const uint32_t cn = (i < 7) ? ((text >> (4 * (i + 1))) & 7u) : (text_next & 7u);
                 uint32_t eqn[W];
                 load_eq<L, W>(lds_lane, cn * 64u, eqn);
-                run_column<L, W>(eq, 8u * b + (uint32_t)i, ncols, Pv, Mv, score, min_score, min_col);
+                if constexpr (L == 1) {
+                    column_step<W, false>(Pv, Mv, eq, 0u, 0u, ph_top, mh_top);
+                } else {
+                    const uint32_t ph_in = from_lower_lane<L>(ph_top, not_first);
+                    const uint32_t mh_in = from_lower_lane<L>(mh_top, not_first);
+                    column_step<W, true>(Pv, Mv, eq, ph_in, mh_in, ph_top, mh_top);
+                }
+                // score row and first minimum (src-api/coreMyersGPU-CC35.cu:205-207,227-228);
+                // only the group's top lane keeps them, at column t - (L-1)
+                score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
+                const uint32_t col = 8u * b + (uint32_t)i - (uint32_t)(L - 1);
+                const bool better = (score < min_score) && (col < ncols);
+                min_col = better ? col : min_col;
+                min_score = better ? score : min_score;
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int k = 0; k < W; k++) eq[k] = eqn[k];
             }
             text = text_next;
-            text_next = sanitize_text(__builtin_amdgcn_alignbit(tw3, tw2, shift));
+            text_next = text_block(tw2, tw3, b + 2);
             tw2 = tw3;
             tw3 = tw4;
         }
 
-        if (valid && (lane % L) == (uint32_t)(L - 1)) {
+        if (valid && gl == (uint32_t)(L - 1)) {
             results[cand] = live ? make_uint2(min_col, (uint32_t)min_score) : make_uint2(0u, kSkipScore);
         }
     }

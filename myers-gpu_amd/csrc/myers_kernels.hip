@@ -80,47 +80,48 @@ struct Shape {
     static constexpr int kTotal = L * W;                // words per candidate
     static constexpr int kSlots = 64 / L;               // candidates per wave
     static constexpr int kRows = L > 1 ? 6 : 5;         // A C G T N (+ zero row)
-    static constexpr int kLdsU4 = kGroups * kRows * 64;
+    // Peq is staged once per distinct query of the wave ("query slot"), not
+    // once per candidate; up to kQSlots queries per pass.  For L = 1 an odd
+    // slot stride spreads the five rows of one query over distinct banks.
+    static constexpr int kQSlotsMax = L == 1 ? 16 : 8;
+    static constexpr int kQSlots = kSlots < kQSlotsMax ? kSlots : kQSlotsMax;
+    static constexpr int kQStride = kQSlots + (L == 1 ? 1 : 0);
+    static constexpr int kRowU4 = kQStride * L;         // uint4 per (group, row)
+    static constexpr int kLdsU4 = kGroups * kRows * kRowU4;
 };
 
-// Stage the Peq words of this lane into LDS, top-justified.  Lane word k
-// holds virtual rows [32g, 32g+32) with g = (lane % L)*W + k; virtual row r
-// is query row r - pad, pad = 32*L*W - m, so lane word k is the funnel of
-// query words lo+k and lo+k+1 (lo = floor((32*g0 - pad)/32)) shifted right
-// by sh = (-pad) mod 32.  Query word j (rows 32j..32j+31) of base c is
-// bitmap[c][j%4] of entry posEntry + j/4; words outside the query read 0.
+// Stage the Peq words of the wave's query slots into LDS, top-justified,
+// with all 64 lanes sharing the work.  Word gw (0 <= gw < L*W) of a slot
+// holds virtual rows [32gw, 32gw+32); virtual row r is query row r - pad,
+// pad = 32*L*W - m, so it is the funnel of query words lo and lo+1
+// (lo = floor((32gw - pad)/32)) shifted right by (-pad) mod 32.  Query word
+// j (rows 32j..32j+31) of base c is bitmap[c][j%4] of entry posEntry + j/4;
+// words outside the query read 0.  Lane gl = gw / W of the group reads word
+// k = gw % W at LDS uint4 ((k/4)*rows + c)*kRowU4 + slot*L + gl, lane k%4.
 template <int L, int W>
-__device__ __forceinline__ void stage_peq(uint4 *lds, const uint32_t *__restrict__ peq,
-                                          uint32_t pos_entry, uint32_t m, uint32_t lane) {
+__device__ __forceinline__ void stage_peq(uint4 *lds, const uint2 *qtab, uint32_t nslots,
+                                          const uint32_t *__restrict__ peq, uint32_t lane) {
     using S = Shape<L, W>;
     uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
-    const int32_t pad = 32 * S::kTotal - (int32_t)m;
-    const int32_t nq = (int32_t)((m + 31u) >> 5);
-    const int32_t b0 = 32 * (int32_t)((lane % L) * W) - pad;
-    const int32_t lo = b0 >> 5;                        // floor division
-    const uint32_t sh = (uint32_t)b0 & 31u;
-    const uint32_t *base = peq + (size_t)pos_entry * 20u;
-#pragma unroll
-    for (int c = 0; c < 5; c++) {
-        uint32_t prev = 0;
-        {
-            const int32_t j = lo;
-            if (j >= 0 && j < nq) prev = base[(uint32_t)(j >> 2) * 20u + (uint32_t)(j & 3) + 4u * c];
-        }
-#pragma unroll
-        for (int k = 0; k < W; k++) {
-            const int32_t j = lo + k + 1;
-            uint32_t next = 0;
-            if (j >= 0 && j < nq) next = base[(uint32_t)(j >> 2) * 20u + (uint32_t)(j & 3) + 4u * c];
-            const uint32_t v = (uint32_t)((((uint64_t)next << 32) | prev) >> sh);
-            l32[(((k >> 2) * S::kRows + c) * 64 + lane) * 4 + (k & 3)] = v;
-            prev = next;
-        }
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    if constexpr (S::kRows > 5) {
-#pragma unroll
-        for (int k = 0; k < W; k++) l32[(((k >> 2) * S::kRows + kZeroRow) * 64 + lane) * 4 + (k & 3)] = 0u;
+    const uint32_t total = nslots * 5u * (uint32_t)S::kTotal;
+    for (uint32_t idx = lane; idx < total; idx += 64u) {
+        const uint32_t gw = idx % (uint32_t)S::kTotal;
+        const uint32_t rest = idx / (uint32_t)S::kTotal;
+        const uint32_t c = rest % 5u;
+        const uint32_t slot = rest / 5u;
+        const uint2 qi = qtab[slot];                   // {posEntry, m}
+        const int32_t pad = 32 * S::kTotal - (int32_t)qi.y;
+        const int32_t nq = (int32_t)((qi.y + 31u) >> 5);
+        const int32_t b0 = 32 * (int32_t)gw - pad;
+        const int32_t lo = b0 >> 5;                    // floor division
+        const uint32_t sh = (uint32_t)b0 & 31u;
+        const uint32_t *base = peq + (size_t)qi.x * 20u + 4u * c;
+        uint32_t wlo = 0, whi = 0;
+        if (lo >= 0 && lo < nq) wlo = base[(uint32_t)(lo >> 2) * 20u + (uint32_t)(lo & 3)];
+        if (lo + 1 >= 0 && lo + 1 < nq) whi = base[(uint32_t)((lo + 1) >> 2) * 20u + (uint32_t)((lo + 1) & 3)];
+        const uint32_t k = gw % (uint32_t)W, gl = gw / (uint32_t)W;
+        const uint32_t u4 = ((k >> 2) * (uint32_t)S::kRows + c) * (uint32_t)S::kRowU4 + slot * (uint32_t)L + gl;
+        l32[u4 * 4u + (k & 3u)] = (uint32_t)((((uint64_t)whi << 32) | wlo) >> sh);
     }
 }
 
@@ -130,7 +131,7 @@ __device__ __forceinline__ void load_eq(const uint4 *lds_lane, uint32_t row_off,
     const uint4 *p = lds_lane + row_off;
 #pragma unroll
     for (int g = 0; g < S::kGroups; g++) {
-        uint4 v = p[g * S::kRows * 64];
+        uint4 v = p[g * S::kRows * S::kRowU4];
         eq[4 * g] = v.x;
         if (4 * g + 1 < W) eq[4 * g + 1] = v.y;
         if (4 * g + 2 < W) eq[4 * g + 2] = v.z;
@@ -197,6 +198,7 @@ template <int L, int W>
 __global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
     using S = Shape<L, W>;
     __shared__ uint4 lds[S::kLdsU4];
+    __shared__ uint2 qtab[S::kQSlots];
 
     const uint32_t lane = threadIdx.x;
     const uint32_t gl = lane % L;                      // lane within the candidate's group
@@ -211,8 +213,17 @@ __global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
     const CandRec *__restrict__ cands = static_cast<const CandRec *>(a.cands);
     const uint2 *__restrict__ qinfo = reinterpret_cast<const uint2 *>(a.qinfo);
     uint2 *__restrict__ results = static_cast<uint2 *>(a.results);
-    const uint4 *lds_lane = lds + lane;
     const uint32_t not_first = gl != 0u ? ~0u : 0u;
+    const uint64_t lanes_upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
+
+    if constexpr (S::kRows > 5) {
+        // the all-zero row read by lanes before their first column
+        uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
+        for (uint32_t i = lane; i < (uint32_t)(S::kGroups * S::kRowU4 * 4); i += 64u) {
+            const uint32_t u4 = i >> 2, g = u4 / S::kRowU4, r = u4 % S::kRowU4;
+            l32[((g * S::kRows + kZeroRow) * S::kRowU4 + r) * 4u + (i & 3u)] = 0u;
+        }
+    }
 
     // Workgroups are dispatched round-robin over the 8 XCDs; renumber them so
     // that each XCD (own L2) takes a contiguous run of chunks, which keeps the
@@ -223,7 +234,7 @@ __global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
     for (uint32_t chunk = first; chunk < nchunks; chunk += gridDim.x) {
         const uint32_t slot = chunk * S::kSlots + lane / L;
         const bool valid = slot < count;
-        uint32_t cand = 0, m = 0, pos_entry = 0;
+        uint32_t cand = 0, m = 0, pos_entry = 0, query = 0xFFFFFFFFu;
         uint64_t pos = 0;
         bool sane = false;
         if (valid) {
@@ -231,6 +242,7 @@ __global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
             cand = a.order ? a.order[s] : s;
             const CandRec cr = cands[cand];
             pos = cr.position;
+            query = cr.query;
             if (cr.query < a.num_queries) {
                 const uint2 qi = qinfo[cr.query];
                 pos_entry = qi.x;
@@ -245,96 +257,116 @@ __global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
         // skip rule: src-api/coreMyersGPU-CC35.cu:142
         const bool live = valid && sane && pos < a.ref_size && (a.ref_size - pos) > (uint64_t)n;
         uint64_t nib0 = 0;
-        uint32_t ncols = 0;
+        uint32_t ncols_live = 0;
         if (live) {
             if (a.mode == 0) {
                 // aligned window: src-api/coreMyersGPU-CC35.cu:132,144
                 nib0 = pos & ~7ull;
-                ncols = 8u * ((n >> 3) + ((n & 7u) ? 2u : 1u));
+                ncols_live = 8u * ((n >> 3) + ((n & 7u) ? 2u : 1u));
             } else {
                 nib0 = pos;
-                ncols = n;
+                ncols_live = n;
             }
         }
-        stage_peq<L, W>(lds, a.peq, pos_entry, live ? m : 0u, lane);
 
-        // Steps: the top lane of a group finishes column ncols-1 at step
-        // ncols-1 + L-1.  Lane gl reads text from nibble nib0 - gl.
-        const uint32_t max_cols = wave_max(ncols);
-        const uint32_t nblocks = (max_cols + (uint32_t)(L - 1) + 7u) >> 3;
-        const int64_t q = (int64_t)nib0 - (int64_t)gl;
-        const int32_t w0 = (int32_t)(q >> 3);            // floor
-        const uint32_t shift = (uint32_t)(q & 7) * 4u;
+        // Query slots: consecutive candidates of one query share a slot (the
+        // callers' batches and the binning keep a query's candidates adjacent).
+        const uint32_t query_below = (uint32_t)__shfl_up((int)query, L);
+        const bool head = (lane < (uint32_t)L || query_below != query) && gl == 0u;
+        const uint64_t heads = __ballot(head);
+        const uint32_t qslot = (uint32_t)__builtin_popcountll(heads & lanes_upto) - 1u;
+        const uint32_t nqslots = (uint32_t)__builtin_popcountll(heads);
 
-        uint32_t Pv[W], Mv[W];
+        // More distinct queries than LDS slots: one pass per kQSlots queries.
+        for (uint32_t base_slot = 0; base_slot < nqslots; base_slot += S::kQSlots) {
+            const bool in_pass = qslot >= base_slot && qslot < base_slot + S::kQSlots;
+            const uint32_t my_slot = in_pass ? qslot - base_slot : 0u;
+            __syncthreads();                             // previous readers of qtab / lds are done
+            if (in_pass && head) qtab[my_slot] = make_uint2(pos_entry, m);
+            __syncthreads();
+            stage_peq<L, W>(lds, qtab, min(nqslots - base_slot, (uint32_t)S::kQSlots), a.peq, lane);
+            __syncthreads();
+            const uint4 *lds_lane = lds + my_slot * L + gl;
+            const uint32_t ncols = in_pass ? ncols_live : 0u;
+
+            // Steps: the top lane of a group finishes column ncols-1 at step
+            // ncols-1 + L-1.  Lane gl reads text from nibble nib0 - gl.
+            const uint32_t max_cols = wave_max(ncols);
+            const uint32_t nblocks = (max_cols + (uint32_t)(L - 1) + 7u) >> 3;
+            const int64_t q = (int64_t)nib0 - (int64_t)gl;
+            const int32_t w0 = (int32_t)(q >> 3);            // floor
+            const uint32_t shift = (uint32_t)(q & 7) * 4u;
+
+            uint32_t Pv[W], Mv[W];
 #pragma unroll
-        for (int k = 0; k < W; k++) {
-            Pv[k] = ~0u;
-            Mv[k] = 0u;
-        }
-        int32_t score = (int32_t)m, min_score = (int32_t)m;
-        uint32_t min_col = 0;
-        uint32_t ph_top = 0, mh_top = 0;
+            for (int k = 0; k < W; k++) {
+                Pv[k] = ~0u;
+                Mv[k] = 0u;
+            }
+            int32_t score = (int32_t)m, min_score = (int32_t)m;
+            uint32_t min_col = 0;
+            uint32_t ph_top = 0, mh_top = 0;
 
-        // Text words stream through registers two blocks (16 columns) ahead
-        // of their use; the Eq words of step t+1 are read from LDS while
-        // step t computes.  Reads outside the stored reference are clamped
-        // to its first / last (all-N guard) word; they only feed steps that
-        // no lane keeps.
-        auto text_word = [&](int32_t k) { return ref[min(max(w0 + k, 0), last_word)]; };
-        auto text_block = [&](uint32_t lo, uint32_t hi, uint32_t b) {
-            uint32_t t = sanitize_text(__builtin_amdgcn_alignbit(hi, lo, shift));
-            if constexpr (L > 1) {
-                // steps before this lane's first column read the zero row
-                const int32_t early = (int32_t)gl - 8 * (int32_t)b;     // such steps in this block
-                if (early > 0) {
-                    const uint32_t low = early >= 8 ? ~0u : ((1u << (4 * early)) - 1u);
-                    t = (t & ~low) | (0x55555555u & low);
+            // Text words stream through registers two blocks (16 columns) ahead
+            // of their use; the Eq words of step t+1 are read from LDS while
+            // step t computes.  Reads outside the stored reference are clamped
+            // to its first / last (all-N guard) word; they only feed steps that
+            // no lane keeps.
+            auto text_word = [&](int32_t k) { return ref[min(max(w0 + k, 0), last_word)]; };
+            auto text_block = [&](uint32_t lo, uint32_t hi, uint32_t b) {
+                uint32_t t = sanitize_text(__builtin_amdgcn_alignbit(hi, lo, shift));
+                if constexpr (L > 1) {
+                    // steps before this lane's first column read the zero row
+                    const int32_t early = (int32_t)gl - 8 * (int32_t)b;     // such steps in this block
+                    if (early > 0) {
+                        const uint32_t low = early >= 8 ? ~0u : ((1u << (4 * early)) - 1u);
+                        t = (t & ~low) | (0x55555555u & low);
+                    }
                 }
-            }
-            return t;
-        };
-        const uint32_t tw0 = text_word(0);
-        const uint32_t tw1 = text_word(1);
-        uint32_t tw2 = text_word(2);
-        uint32_t tw3 = text_word(3);
-        uint32_t text = text_block(tw0, tw1, 0);
-        uint32_t text_next = text_block(tw1, tw2, 1);
-        uint32_t eq[W];
-        load_eq<L, W>(lds_lane, (text & 7u) * 64u, eq);
-        for (uint32_t b = 0; b < nblocks; b++) {
-            const uint32_t tw4 = text_word((int32_t)b + 4);
+                return t;
+            };
+            const uint32_t tw0 = text_word(0);
+            const uint32_t tw1 = text_word(1);
+            uint32_t tw2 = text_word(2);
+            uint32_t tw3 = text_word(3);
+            uint32_t text = text_block(tw0, tw1, 0);
+            uint32_t text_next = text_block(tw1, tw2, 1);
+            uint32_t eq[W];
+            load_eq<L, W>(lds_lane, (text & 7u) * S::kRowU4, eq);
+            for (uint32_t b = 0; b < nblocks; b++) {
+                const uint32_t tw4 = text_word((int32_t)b + 4);
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const uint32_t cn = (i < 7) ? ((text >> (4 * (i + 1))) & 7u) : (text_next & 7u);
-                uint32_t eqn[W];
-                load_eq<L, W>(lds_lane, cn * 64u, eqn);
-                if constexpr (L == 1) {
-                    column_step<W, false>(Pv, Mv, eq, 0u, 0u, ph_top, mh_top);
-                } else {
-                    const uint32_t ph_in = from_lower_lane<L>(ph_top, not_first);
-                    const uint32_t mh_in = from_lower_lane<L>(mh_top, not_first);
-                    column_step<W, true>(Pv, Mv, eq, ph_in, mh_in, ph_top, mh_top);
+                for (int i = 0; i < 8; i++) {
+                    const uint32_t cn = (i < 7) ? ((text >> (4 * (i + 1))) & 7u) : (text_next & 7u);
+                    uint32_t eqn[W];
+                    load_eq<L, W>(lds_lane, cn * S::kRowU4, eqn);
+                    if constexpr (L == 1) {
+                        column_step<W, false>(Pv, Mv, eq, 0u, 0u, ph_top, mh_top);
+                    } else {
+                        const uint32_t ph_in = from_lower_lane<L>(ph_top, not_first);
+                        const uint32_t mh_in = from_lower_lane<L>(mh_top, not_first);
+                        column_step<W, true>(Pv, Mv, eq, ph_in, mh_in, ph_top, mh_top);
+                    }
+                    // score row and first minimum (src-api/coreMyersGPU-CC35.cu:205-207,227-228);
+                    // only the group's top lane keeps them, at column t - (L-1)
+                    score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
+                    const uint32_t col = 8u * b + (uint32_t)i - (uint32_t)(L - 1);
+                    const bool better = (score < min_score) && (col < ncols);
+                    min_col = better ? col : min_col;
+                    min_score = better ? score : min_score;
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int k = 0; k < W; k++) eq[k] = eqn[k];
                 }
-                // score row and first minimum (src-api/coreMyersGPU-CC35.cu:205-207,227-228);
-                // only the group's top lane keeps them, at column t - (L-1)
-                score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
-                const uint32_t col = 8u * b + (uint32_t)i - (uint32_t)(L - 1);
-                const bool better = (score < min_score) && (col < ncols);
-                min_col = better ? col : min_col;
-                min_score = better ? score : min_score;
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int k = 0; k < W; k++) eq[k] = eqn[k];
+                text = text_next;
+                text_next = text_block(tw2, tw3, b + 2);
+                tw2 = tw3;
+                tw3 = tw4;
             }
-            text = text_next;
-            text_next = text_block(tw2, tw3, b + 2);
-            tw2 = tw3;
-            tw3 = tw4;
-        }
 
-        if (valid && gl == (uint32_t)(L - 1)) {
-            results[cand] = live ? make_uint2(min_col, (uint32_t)min_score) : make_uint2(0u, kSkipScore);
+            if (in_pass && valid && gl == (uint32_t)(L - 1)) {
+                results[cand] = live ? make_uint2(min_col, (uint32_t)min_score) : make_uint2(0u, kSkipScore);
+            }
         }
     }
 }

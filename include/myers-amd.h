@@ -48,6 +48,9 @@ typedef enum {
 #define MYERS_ENGINE_PLAIN      (1u << 4)  /* one launch per length bin */
 #define MYERS_ENGINE_PERSISTENT (2u << 4)  /* one persistent launch over a length-binned queue */
 #define MYERS_ENGINE_MASK       (3u << 4)
+/* Capture the batch's on-device pipeline (binning + kernels) into a HIP
+ * graph once per batch shape and replay it (extension). */
+#define MYERS_USE_GRAPH         (1u << 6)
 
 /* Longest query supported, in bases (extension: the reference stops at 4096,
  * src-api/myersGPU.c:366,406-410). */
@@ -78,6 +81,7 @@ typedef struct {
     double   kernelMsTotal;         /* sum over batches of their kernel time */
     uint64_t kernelLaunchesTotal;   /* verification launches */
     uint64_t batchesTotal;          /* timed batches */
+    uint64_t graphCaptures;         /* graphs captured (MYERS_USE_GRAPH) */
 } myersStats_t;
 
 void myersDefaultConfig(myersConfig_t *cfg);
@@ -150,6 +154,74 @@ uint32_t myersNumVariants(void);
 int      myersGetVariantInfo(void *myersBuffer, uint32_t variant, myersVariantInfo_t *out);
 int      myersGetBinTable(void *myersBuffer, uint8_t *bin, uint32_t maxWords);
 int      myersSetBinTable(void *myersBuffer, const uint8_t *bin, uint32_t maxWords);
+/* Chunks (of candidatesPerWave candidates) a persistent workgroup takes per
+ * pull from its bin's queue (1..64). */
+int      myersSetChunkFactor(void *myersBuffer, uint32_t variant, uint32_t chunks);
+int      myersGetChunkFactor(void *myersBuffer, uint32_t variant, uint32_t *chunks);
+/* Load an autotuner table: text lines "maxWords lanes words chunkFactor"
+ * (maxWords ascending; '#' starts a comment).  Queries of w words use the
+ * first line with maxWords >= w; longer ones keep the built-in choice.
+ * initMyers loads $MYERS_TUNING_FILE, else tuning/gfx950.tsv next to the
+ * library's directory, if present. */
+int      myersLoadTuning(void *myersBuffer, const char *path);
+
+/* ---- Serving: continuous batching (extension) -------------------------
+ * A server thread owns the buffers handed to it (each is one batch slot with
+ * its own stream).  Requests are admitted FIFO into a forming batch while the
+ * batch fits a slot and the device bytes of admitted, unfinished requests
+ * stay within hbmBudgetBytes; a batch is launched as soon as a slot is free,
+ * so requests arriving while others run join the next batch and batches run
+ * concurrently on the device.  Results are copied to each request's output
+ * when its batch completes; every candidate's result is the same as when its
+ * request is verified alone. */
+typedef struct {
+    uint32_t structSize;
+    uint32_t reserved;
+    float    distance;          /* window n = (uint32)(m*(1+2*distance)) for every request */
+    uint32_t flags;             /* MYERS_WINDOW_* | MYERS_ENGINE_* */
+    uint64_t hbmBudgetBytes;    /* admission budget (0 = 200e9 bytes) */
+    uint32_t pollMicros;        /* completion poll period while batches run (0 = 20) */
+    uint32_t maxBatchCandidates;/* cap per batch (0 = slot capacity) */
+} myersServeConfig_t;
+
+typedef struct {
+    const qryEntry_t *peq;
+    const qryInfo_t  *qinfo;    /* posEntry relative to this request's peq */
+    const candInfo_t *cands;    /* query relative to this request's qinfo */
+    resEntry_t       *results;  /* numCandidates results, written on completion */
+    uint32_t          numPEQEntries;
+    uint32_t          numQueries;
+    uint32_t          numCandidates;
+    uint32_t          reserved;
+} myersRequest_t;
+
+typedef struct {
+    uint32_t structSize;
+    uint32_t reserved;
+    uint64_t requests;          /* completed */
+    uint64_t batches;           /* launched */
+    uint64_t candidates;        /* verified */
+    uint64_t maxBatchRequests;
+    double   busyMs;            /* sum of batch kernel time */
+} myersServeStats_t;
+
+void myersServeDefaultConfig(myersServeConfig_t *cfg);
+/* Start a server over numBuffers buffers of one initMyers call (the server
+ * uses them exclusively until myersServeStop). */
+int  myersServeStart(void **server, void **myersBuffers, uint32_t numBuffers, const myersServeConfig_t *cfg);
+/* Queue a request; the input arrays must stay valid until the request is
+ * admitted (myersServeWait returns), results until it completes. */
+int  myersServeSubmit(void *server, const myersRequest_t *req, uint64_t *ticket);
+/* Block until the request completes; latencyMs gets submit-to-completion time. */
+int  myersServeWait(void *server, uint64_t ticket, double *latencyMs);
+int  myersServeGetStats(void *server, myersServeStats_t *out);
+/* Drain every queued request, stop the server thread, release the buffers. */
+int  myersServeStop(void *server);
+/* Open-loop driver: submit request i at arrivalNs[i] (ns after the call
+ * starts, ascending) from a submitter thread, whatever the server's progress;
+ * wait for all; latencyMs[i] = completion - arrival. */
+int  myersServeRunOpenLoop(void *server, uint32_t numRequests, const myersRequest_t *reqs,
+                           const uint64_t *arrivalNs, double *latencyMs, double *wallMs);
 
 #ifdef __cplusplus
 }

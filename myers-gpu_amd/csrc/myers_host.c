@@ -22,6 +22,8 @@
  * of the packed reference in HBM (the reference freed it once per buffer,
  * myersGPU.c:661).
  */
+#define _GNU_SOURCE
+#include <dlfcn.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -30,6 +32,7 @@
 
 #include "myers-interface.h"
 #include "myers-amd.h"
+#include "myers_internal.h"
 #include "myers_kernels.h"
 
 #define GUARD_WORDS      64u            /* all-N words after the reference */
@@ -38,6 +41,8 @@
 #define STAGE_BYTES      (64u << 20)    /* pinned staging for the reference upload */
 #define DEFAULT_BUFFER_CAP (2ull << 30)
 #define EV_RING          32u            /* kernel-timing event pairs per buffer */
+#define SIDE_STREAMS     4u             /* fork width of a persistent launch */
+#define WORK_WORDS       (6u * MYERS_MAX_VARIANTS + 1u)
 
 typedef struct myers_ctx {
     int       device;
@@ -48,9 +53,12 @@ typedef struct myers_ctx {
     uint64_t  ref_size;          /* bases */
     uint32_t  num_variants;
     myers_variant_desc_t variants[MYERS_MAX_VARIANTS];
-    int       occupancy[MYERS_MAX_VARIANTS];
+    int       occupancy[MYERS_MAX_VARIANTS];       /* plain kernel, workgroups per CU */
+    int       occupancy_p[MYERS_MAX_VARIANTS];     /* persistent kernel */
+    uint32_t  chunk_factor[MYERS_MAX_VARIANTS];    /* persistent: chunks per queue pull */
     uint8_t   bin_of_words[MAX_QUERY_WORDS + 1];
     uint8_t  *d_bin_of_words;
+    uint64_t  tune_gen;          /* bumped when the bin table or chunk factors change */
 } myers_ctx_t;
 
 typedef struct myers_buffer {
@@ -70,7 +78,7 @@ typedef struct myers_buffer {
     resEntry_t  *h_results;
     resEntry_t  *d_results;
     uint32_t    *d_order;
-    uint32_t    *d_work;          /* counts | cursors | ranges(2x) | err */
+    uint32_t    *d_work;          /* counts | cursors | ranges(2x) | queue heads | spare | err */
     uint32_t    *h_err;           /* pinned copy of the binning error flag */
     hipEvent_t   ev_batch_start, ev_batch_end;
     /* HIP events around the verification launches of each batch, in a ring
@@ -79,6 +87,15 @@ typedef struct myers_buffer {
     hipEvent_t   ev_kernel[EV_RING][2];
     uint32_t     ev_launches[EV_RING];
     uint32_t     ev_head, ev_pending;
+    /* persistent engine: variants run on forked streams joined by events */
+    hipStream_t  side[SIDE_STREAMS];
+    hipEvent_t   ev_fork, ev_join[SIDE_STREAMS];
+    /* steady-state graph of the last device-resident batch shape */
+    hipGraphExec_t graph_exec;
+    uint64_t     graph_key[10];
+    int          graph_valid;
+    int          graph_binned;
+    uint32_t     graph_launches;
     int          exit_on_error;
     int          last_error;
     int          pending;         /* a batch was enqueued and not yet received */
@@ -86,6 +103,8 @@ typedef struct myers_buffer {
     uint32_t     num_candidates;
     myersStats_t stats;
 } myers_buffer_t;
+
+static void load_default_tuning(myers_ctx_t *ctx);
 
 /* ------------------------------------------------------------------ errors */
 
@@ -262,6 +281,12 @@ static void free_buffer(myers_buffer_t *b)
         if (b->ev_kernel[i][0]) hipEventDestroy(b->ev_kernel[i][0]);
         if (b->ev_kernel[i][1]) hipEventDestroy(b->ev_kernel[i][1]);
     }
+    if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+    for (uint32_t i = 0; i < SIDE_STREAMS; i++) {
+        if (b->side[i]) hipStreamDestroy(b->side[i]);
+        if (b->ev_join[i]) hipEventDestroy(b->ev_join[i]);
+    }
+    if (b->ev_fork) hipEventDestroy(b->ev_fork);
     if (b->stream) hipStreamDestroy(b->stream);
     free(b);
 }
@@ -283,7 +308,12 @@ static int init_buffer(myers_buffer_t *b)
     HIP_TRY(hipMalloc((void **)&b->d_cands, ca_b));
     HIP_TRY(hipMalloc((void **)&b->d_results, re_b));
     HIP_TRY(hipMalloc((void **)&b->d_order, (size_t)b->max_candidates * sizeof(uint32_t)));
-    HIP_TRY(hipMalloc((void **)&b->d_work, (5u * MYERS_MAX_VARIANTS + 1u) * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc((void **)&b->d_work, WORK_WORDS * sizeof(uint32_t)));
+    for (uint32_t i = 0; i < SIDE_STREAMS; i++) {
+        HIP_TRY(hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&b->ev_join[i], hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
     HIP_TRY(hipEventCreate(&b->ev_batch_start));
     HIP_TRY(hipEventCreate(&b->ev_batch_end));
     for (uint32_t i = 0; i < EV_RING; i++) {
@@ -314,6 +344,10 @@ static int create_context(myers_ctx_t **out, int device)
         int occ = 0;
         if (myers_variant_occupancy(v, &occ) != 0 || occ <= 0) occ = 1;
         ctx->occupancy[v] = occ;
+        occ = 0;
+        if (myers_variant_occupancy_persistent(v, &occ) != 0 || occ <= 0) occ = 1;
+        ctx->occupancy_p[v] = occ;
+        ctx->chunk_factor[v] = 1u;
     }
     default_bin_table(ctx);
     if (hipMalloc((void **)&ctx->d_bin_of_words, sizeof(ctx->bin_of_words)) != hipSuccess) {
@@ -326,6 +360,7 @@ static int create_context(myers_ctx_t **out, int device)
         free(ctx);
         return MYERS_E_HIP;
     }
+    load_default_tuning(ctx);
     *out = ctx;
     return MYERS_SUCCESS;
 }
@@ -536,79 +571,185 @@ static int timing_end(myers_buffer_t *b, uint32_t slot, uint32_t launches)
     return MYERS_SUCCESS;
 }
 
-/* Enqueue the verification of num_cands candidates on b->stream. */
-static int enqueue_verify(myers_buffer_t *b, const void *d_peq, uint32_t num_entries, const void *d_qinfo,
-                          const void *d_cands, uint32_t num_cands, float distance, uint32_t flags,
-                          void *d_results, const qryInfo_t *h_qinfo, uint32_t num_queries)
+typedef struct {
+    const void *d_peq, *d_qinfo, *d_cands;
+    void       *d_results;
+    uint32_t    num_entries, num_queries, num_cands, flags;
+    float       distance;
+} batch_t;
+
+static uint32_t resolve_engine(uint32_t flags, const bin_scan_t *scan)
+{
+    uint32_t engine = flags & MYERS_ENGINE_MASK;
+    if (engine == MYERS_ENGINE_AUTO)
+        engine = (scan->num_present == 1 && !scan->too_long) ? MYERS_ENGINE_PLAIN : MYERS_ENGINE_PERSISTENT;
+    return engine;
+}
+
+/* Variants present, most words first (longest candidates start first). */
+static uint32_t present_by_cost(const myers_ctx_t *ctx, uint64_t present, uint32_t *out)
+{
+    uint32_t n = 0;
+    for (uint32_t v = 0; v < ctx->num_variants; v++)
+        if (present & (1ull << v)) out[n++] = v;
+    for (uint32_t i = 1; i < n; i++) {
+        uint32_t x = out[i], j = i;
+        while (j > 0 && ctx->variants[out[j - 1]].total_words < ctx->variants[x].total_words) {
+            out[j] = out[j - 1];
+            j--;
+        }
+        out[j] = x;
+    }
+    return n;
+}
+
+/* Enqueue the whole on-device pipeline of one batch on b->stream (no timing
+ * events, so that it can be captured into a graph). */
+static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan_t *scan)
 {
     myers_ctx_t *ctx = b->ctx;
-    const uint32_t mode = flags & 1u;
-    bin_scan_t scan;
-    if (h_qinfo) {
-        scan_qinfo(ctx, h_qinfo, num_queries, &scan);
-    } else {
-        memset(&scan, 0, sizeof(scan));
-        scan.present = ctx->num_variants >= 64 ? ~0ull : ((1ull << ctx->num_variants) - 1u);
-        scan.num_present = ctx->num_variants;
-        scan.too_long = 1;   /* unknown: let the device check */
-    }
     myers_verify_args_t a;
     memset(&a, 0, sizeof(a));
     a.ref = ctx->d_ref;
     a.ref_words = ctx->ref_words_alloc;
     a.ref_size = ctx->ref_size;
-    a.peq = (const uint32_t *)d_peq;
-    a.qinfo = (const uint32_t *)d_qinfo;
-    a.num_peq_entries = num_entries;
-    a.num_queries = num_queries;
-    a.cands = d_cands;
-    a.mode = mode;
-    a.distance = distance;
-    a.results = d_results;
+    a.peq = (const uint32_t *)bt->d_peq;
+    a.qinfo = (const uint32_t *)bt->d_qinfo;
+    a.num_peq_entries = bt->num_entries;
+    a.num_queries = bt->num_queries;
+    a.cands = bt->d_cands;
+    a.mode = bt->flags & 1u;
+    a.distance = bt->distance;
+    a.results = bt->d_results;
 
+    const uint32_t engine = resolve_engine(bt->flags, scan);
     b->stats.launches = 0;
     b->stats.variantsUsed = 0;
     b->binned = 0;
-    uint32_t slot = 0;
     int rc;
-    if (num_cands == 0) return MYERS_SUCCESS;
-    if (scan.num_present == 1 && !scan.too_long) {
-        const uint32_t v = scan.single;
-        a.order = NULL;
-        a.bin_ranges = NULL;
+    if (bt->num_cands == 0) return MYERS_SUCCESS;
+    if (engine == MYERS_ENGINE_PLAIN && scan->num_present == 1 && !scan->too_long) {
+        const uint32_t v = scan->single;
         a.start = 0;
-        a.count = num_cands;
-        if ((rc = timing_begin(b, &slot))) return rc;
-        rc = myers_launch_verify(v, &a, grid_for(ctx, v, num_cands, 0), b->stream);
+        a.count = bt->num_cands;
+        rc = myers_launch_verify(v, &a, grid_for(ctx, v, bt->num_cands, 0), 0, b->stream);
         if (rc) return hip_fail((hipError_t)rc, "verify launch", __FILE__, __LINE__);
         b->stats.launches = 1;
         b->stats.variantsUsed = 1;
-        return timing_end(b, slot, 1u);
+        return MYERS_SUCCESS;
     }
     /* device-side binning */
     uint32_t *counts = b->d_work;
     uint32_t *cursors = counts + MYERS_MAX_VARIANTS;
     uint32_t *ranges = cursors + MYERS_MAX_VARIANTS;
-    uint32_t *err = ranges + 2u * MYERS_MAX_VARIANTS;
-    rc = myers_launch_binning((const uint32_t *)d_qinfo, num_queries, d_cands, num_cands, ctx->d_bin_of_words,
-                                  max_supported_words(ctx), ctx->num_variants, counts, cursors, ranges,
-                                  b->d_order, err, d_results, b->stream);
+    uint32_t *heads = ranges + 2u * MYERS_MAX_VARIANTS;
+    uint32_t *err = b->d_work + WORK_WORDS - 1u;
+    rc = myers_launch_binning((const uint32_t *)bt->d_qinfo, bt->num_queries, bt->d_cands, bt->num_cands,
+                              ctx->d_bin_of_words, max_supported_words(ctx), ctx->num_variants, counts, cursors,
+                              ranges, b->d_order, err, bt->d_results, heads, b->stream);
     if (rc) return hip_fail((hipError_t)rc, "binning launch", __FILE__, __LINE__);
     HIP_TRY(hipMemcpyAsync(b->h_err, err, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
     b->binned = 1;
     a.order = b->d_order;
     a.bin_ranges = ranges;
-    if ((rc = timing_begin(b, &slot))) return rc;
-    /* longest queries first: they have the fewest workgroups per launch */
-    for (int v = (int)ctx->num_variants - 1; v >= 0; v--) {
-        if (!(scan.present & (1ull << v))) continue;
-        a.variant = (uint32_t)v;
-        rc = myers_launch_verify((uint32_t)v, &a, grid_for(ctx, (uint32_t)v, num_cands, 1), b->stream);
-        if (rc) return hip_fail((hipError_t)rc, "verify launch", __FILE__, __LINE__);
-        b->stats.launches++;
-        b->stats.variantsUsed++;
+    a.queue_heads = heads;
+    uint32_t order[MYERS_MAX_VARIANTS];
+    const uint32_t n = present_by_cost(ctx, scan->present, order);
+    if (engine == MYERS_ENGINE_PLAIN) {
+        /* one launch per length bin, in turn */
+        for (uint32_t i = 0; i < n; i++) {
+            a.variant = order[i];
+            rc = myers_launch_verify(order[i], &a, grid_for(ctx, order[i], bt->num_cands, 1), 0, b->stream);
+            if (rc) return hip_fail((hipError_t)rc, "verify launch", __FILE__, __LINE__);
+        }
+    } else {
+        /* persistent: every bin's resident grid drains its queue; the bins run
+         * on forked streams so that one bin's tail overlaps the next bin */
+        const uint32_t width = n < SIDE_STREAMS ? n : SIDE_STREAMS;
+        HIP_TRY(hipEventRecord(b->ev_fork, b->stream));
+        for (uint32_t j = 0; j < width; j++) HIP_TRY(hipStreamWaitEvent(b->side[j], b->ev_fork, 0));
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t v = order[i];
+            a.variant = v;
+            a.chunk_factor = ctx->chunk_factor[v];
+            uint64_t grid = (uint64_t)ctx->num_cus * (uint64_t)ctx->occupancy_p[v];
+            rc = myers_launch_verify(v, &a, (uint32_t)grid, 1, b->side[i % width]);
+            if (rc) return hip_fail((hipError_t)rc, "persistent launch", __FILE__, __LINE__);
+        }
+        for (uint32_t j = 0; j < width; j++) {
+            HIP_TRY(hipEventRecord(b->ev_join[j], b->side[j]));
+            HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_join[j], 0));
+        }
     }
-    return timing_end(b, slot, b->stats.launches);
+    b->stats.launches = n;
+    b->stats.variantsUsed = n;
+    return MYERS_SUCCESS;
+}
+
+static void make_scan(myers_ctx_t *ctx, const qryInfo_t *h_qinfo, uint32_t num_queries, bin_scan_t *scan)
+{
+    if (h_qinfo) {
+        scan_qinfo(ctx, h_qinfo, num_queries, scan);
+    } else {
+        memset(scan, 0, sizeof(*scan));
+        scan->present = ctx->num_variants >= 64 ? ~0ull : ((1ull << ctx->num_variants) - 1u);
+        scan->num_present = ctx->num_variants;
+        scan->too_long = 1;   /* unknown: let the device check */
+    }
+}
+
+/* Enqueue the verification of a batch on b->stream, timed with HIP events;
+ * with MYERS_USE_GRAPH the pipeline is captured once per batch shape and
+ * replayed as a graph. */
+static int enqueue_verify(myers_buffer_t *b, const batch_t *bt, const qryInfo_t *h_qinfo)
+{
+    bin_scan_t scan;
+    make_scan(b->ctx, h_qinfo, bt->num_queries, &scan);
+    uint32_t slot = 0;
+    int rc;
+    if (!(bt->flags & MYERS_USE_GRAPH) || bt->num_cands == 0) {
+        if ((rc = timing_begin(b, &slot))) return rc;
+        if ((rc = enqueue_pipeline(b, bt, &scan))) return rc;
+        return timing_end(b, slot, b->stats.launches);
+    }
+    uint32_t dist_bits;
+    memcpy(&dist_bits, &bt->distance, sizeof(dist_bits));
+    const uint64_t key[10] = {(uint64_t)(uintptr_t)bt->d_peq, (uint64_t)(uintptr_t)bt->d_qinfo,
+                              (uint64_t)(uintptr_t)bt->d_cands, (uint64_t)(uintptr_t)bt->d_results,
+                              ((uint64_t)bt->num_entries << 32) | bt->num_queries,
+                              ((uint64_t)bt->num_cands << 32) | bt->flags,
+                              (uint64_t)dist_bits, scan.present,
+                              (uint64_t)scan.too_long, b->ctx->tune_gen};
+    if (!b->graph_valid || memcmp(key, b->graph_key, sizeof(key)) != 0) {
+        if (b->graph_exec) {
+            hipGraphExecDestroy(b->graph_exec);
+            b->graph_exec = NULL;
+        }
+        b->graph_valid = 0;
+        hipGraph_t graph = NULL;
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_pipeline(b, bt, &scan);
+        hipError_t e = hipStreamEndCapture(b->stream, &graph);
+        if (rc) {
+            if (graph) hipGraphDestroy(graph);
+            return rc;
+        }
+        HIP_TRY(e);
+        e = hipGraphInstantiate(&b->graph_exec, graph, NULL, NULL, 0);
+        hipGraphDestroy(graph);
+        HIP_TRY(e);
+        memcpy(b->graph_key, key, sizeof(key));
+        b->graph_valid = 1;
+        b->graph_binned = b->binned;
+        b->graph_launches = b->stats.launches;
+        b->stats.graphCaptures++;
+    }
+    if ((rc = timing_begin(b, &slot))) return rc;
+    HIP_TRY(hipGraphLaunch(b->graph_exec, b->stream));
+    b->binned = b->graph_binned;
+    b->stats.launches = b->graph_launches;
+    b->stats.variantsUsed = b->graph_launches;
+    return timing_end(b, slot, b->graph_launches);
 }
 
 static uint64_t nominal_cells(const qryInfo_t *qinfo, const candInfo_t *cands, uint32_t nc, float d)
@@ -627,7 +768,7 @@ int myersSendEx(void *myersBuffer, float distance, uint32_t numPEQEntries, uint3
 {
     myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
     if (b == NULL) return MYERS_E_INVALID_ARGUMENT;
-    if (flags & ~(1u | MYERS_ENGINE_MASK)) return (b->last_error = MYERS_E_INVALID_ARGUMENT);
+    if (flags & ~(1u | MYERS_ENGINE_MASK | MYERS_USE_GRAPH)) return (b->last_error = MYERS_E_INVALID_ARGUMENT);
     if (numPEQEntries > b->max_peq_entries || numQueries > b->max_queries || numCandidates > b->max_candidates)
         return (b->last_error = MYERS_E_CAPACITY);
     int rc = hip_fail(hipSetDevice(b->ctx->device), "hipSetDevice", __FILE__, __LINE__);
@@ -646,8 +787,9 @@ int myersSendEx(void *myersBuffer, float distance, uint32_t numPEQEntries, uint3
         SEND_TRY(hipMemcpyAsync(b->d_qinfo, b->h_qinfo, (size_t)numQueries * sizeof(qryInfo_t), hipMemcpyHostToDevice, s));
     if (numCandidates)
         SEND_TRY(hipMemcpyAsync(b->d_cands, b->h_cands, (size_t)numCandidates * sizeof(candInfo_t), hipMemcpyHostToDevice, s));
-    rc = enqueue_verify(b, b->d_peq, numPEQEntries, b->d_qinfo, b->d_cands, numCandidates, distance, flags,
-                        b->d_results, b->h_qinfo, numQueries);
+    const batch_t bt = {b->d_peq, b->d_qinfo, b->d_cands, b->d_results, numPEQEntries, numQueries,
+                        numCandidates, flags, distance};
+    rc = enqueue_verify(b, &bt, b->h_qinfo);
     if (rc) return (b->last_error = rc);
     if (numCandidates)
         SEND_TRY(hipMemcpyAsync(b->h_results, b->d_results, (size_t)numCandidates * sizeof(resEntry_t), hipMemcpyDeviceToHost, s));
@@ -677,6 +819,15 @@ static int finish_batch(myers_buffer_t *b, int with_batch_events)
     if (b->binned && (*b->h_err & 1u)) return MYERS_E_QUERY_TOO_LONG;
     if (b->binned && (*b->h_err & 2u)) return MYERS_E_INVALID_ARGUMENT;
     return MYERS_SUCCESS;
+}
+
+int myers_buffer_ready(void *myersBuffer)
+{
+    myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
+    hipError_t e = hipStreamQuery(b->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return -(int)e;
 }
 
 int myersReceiveEx(void *myersBuffer)
@@ -710,14 +861,16 @@ int myersVerifyDevice(void *myersBuffer, const void *d_peq, uint32_t numPEQEntri
 {
     myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
     if (b == NULL) return MYERS_E_INVALID_ARGUMENT;
-    if ((flags & ~(1u | MYERS_ENGINE_MASK)) || (numCandidates && (!d_peq || !d_qinfo || !d_candidates || !d_results)))
+    if ((flags & ~(1u | MYERS_ENGINE_MASK | MYERS_USE_GRAPH)) ||
+        (numCandidates && (!d_peq || !d_qinfo || !d_candidates || !d_results)))
         return (b->last_error = MYERS_E_INVALID_ARGUMENT);
     if (numCandidates > b->max_candidates) return (b->last_error = MYERS_E_CAPACITY);
     int rc = hip_fail(hipSetDevice(b->ctx->device), "hipSetDevice", __FILE__, __LINE__);
     if (rc) return (b->last_error = rc);
     *b->h_err = 0;
-    rc = enqueue_verify(b, d_peq, numPEQEntries, d_qinfo, d_candidates, numCandidates, distance, flags,
-                        d_results, h_qinfo, numQueries);
+    const batch_t bt = {d_peq, d_qinfo, d_candidates, d_results, numPEQEntries, numQueries, numCandidates,
+                        flags, distance};
+    rc = enqueue_verify(b, &bt, h_qinfo);
     b->stats.cells = 0;
     b->stats.candidates = numCandidates;
     b->pending = (rc == MYERS_SUCCESS);
@@ -814,6 +967,102 @@ int myersSetBinTable(void *myersBuffer, const uint8_t *bin, uint32_t maxWords)
     rc = hip_fail(hipDeviceSynchronize(), "hipDeviceSynchronize", __FILE__, __LINE__);
     if (rc) return rc;
     for (uint32_t w = 0; w <= maxWords; w++) ctx->bin_of_words[w] = bin[w];
+    ctx->tune_gen++;
     return hip_fail(hipMemcpy(ctx->d_bin_of_words, ctx->bin_of_words, sizeof(ctx->bin_of_words),
                               hipMemcpyHostToDevice), "hipMemcpy(bin table)", __FILE__, __LINE__);
+}
+
+int myersSetChunkFactor(void *myersBuffer, uint32_t variant, uint32_t chunks)
+{
+    myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
+    if (b == NULL || variant >= b->ctx->num_variants || chunks < 1u || chunks > 64u) return MYERS_E_INVALID_ARGUMENT;
+    b->ctx->chunk_factor[variant] = chunks;
+    b->ctx->tune_gen++;   /* cached graphs baked in the old factor */
+    return MYERS_SUCCESS;
+}
+
+int myersGetChunkFactor(void *myersBuffer, uint32_t variant, uint32_t *chunks)
+{
+    myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
+    if (b == NULL || chunks == NULL || variant >= b->ctx->num_variants) return MYERS_E_INVALID_ARGUMENT;
+    *chunks = b->ctx->chunk_factor[variant];
+    return MYERS_SUCCESS;
+}
+
+static int find_variant(const myers_ctx_t *ctx, uint32_t lanes, uint32_t words)
+{
+    for (uint32_t v = 0; v < ctx->num_variants; v++)
+        if (ctx->variants[v].lanes == lanes && ctx->variants[v].words == words) return (int)v;
+    return -1;
+}
+
+/* Apply a tuning table to a context (not yet shared with running work). */
+static int load_tuning_ctx(myers_ctx_t *ctx, const char *path)
+{
+    FILE *fp = fopen(path, "r");
+    if (fp == NULL) return MYERS_E_OPENING_FILE;
+    uint8_t bin[MAX_QUERY_WORDS + 1];
+    memcpy(bin, ctx->bin_of_words, sizeof(bin));
+    uint32_t cf[MYERS_MAX_VARIANTS];
+    memcpy(cf, ctx->chunk_factor, sizeof(cf));
+    char line[256];
+    uint32_t lo = 0;
+    int rc = MYERS_SUCCESS;
+    while (fgets(line, sizeof(line), fp)) {
+        char *hash = strchr(line, '#');
+        if (hash) *hash = 0;
+        unsigned mw, lanes, words, chunks;
+        int got = sscanf(line, "%u %u %u %u", &mw, &lanes, &words, &chunks);
+        if (got <= 0) continue;
+        if (got != 4 || mw < lo || mw > MAX_QUERY_WORDS || chunks < 1u || chunks > 64u) { rc = MYERS_E_READING_FILE; break; }
+        int v = find_variant(ctx, lanes, words);
+        if (v < 0 || ctx->variants[v].total_words < mw) { rc = MYERS_E_READING_FILE; break; }
+        for (uint32_t w = lo; w <= mw; w++) bin[w] = (uint8_t)v;
+        cf[v] = chunks;
+        lo = mw + 1u;
+    }
+    fclose(fp);
+    if (rc) return rc;
+    memcpy(ctx->bin_of_words, bin, sizeof(bin));
+    memcpy(ctx->chunk_factor, cf, sizeof(cf));
+    ctx->tune_gen++;
+    return hip_fail(hipMemcpy(ctx->d_bin_of_words, ctx->bin_of_words, sizeof(ctx->bin_of_words),
+                              hipMemcpyHostToDevice), "hipMemcpy(bin table)", __FILE__, __LINE__);
+}
+
+int myersLoadTuning(void *myersBuffer, const char *path)
+{
+    myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
+    if (b == NULL || path == NULL) return MYERS_E_INVALID_ARGUMENT;
+    int rc = hip_fail(hipSetDevice(b->ctx->device), "hipSetDevice", __FILE__, __LINE__);
+    if (rc) return rc;
+    rc = hip_fail(hipDeviceSynchronize(), "hipDeviceSynchronize", __FILE__, __LINE__);
+    if (rc) return rc;
+    return load_tuning_ctx(b->ctx, path);
+}
+
+/* $MYERS_TUNING_FILE, else <library dir>/../tuning/gfx950.tsv; silently
+ * keeps the built-in table when neither exists. */
+static void load_default_tuning(myers_ctx_t *ctx)
+{
+    const char *env = getenv("MYERS_TUNING_FILE");
+    if (env && *env) {
+        if (load_tuning_ctx(ctx, env) != MYERS_SUCCESS)
+            fprintf(stderr, "Myers - warning: could not apply tuning file %s\n", env);
+        return;
+    }
+    Dl_info info;
+    if (dladdr((void *)&load_default_tuning, &info) == 0 || info.dli_fname == NULL) return;
+    char path[4096];
+    size_t n = strlen(info.dli_fname);
+    if (n + 32 >= sizeof(path)) return;
+    memcpy(path, info.dli_fname, n + 1);
+    char *slash = strrchr(path, '/');
+    if (slash == NULL) return;
+    strcpy(slash, "/../tuning/gfx950.tsv");
+    FILE *fp = fopen(path, "r");
+    if (fp == NULL) return;
+    fclose(fp);
+    if (load_tuning_ctx(ctx, path) != MYERS_SUCCESS)
+        fprintf(stderr, "Myers - warning: could not apply tuning file %s\n", path);
 }

@@ -40,8 +40,9 @@ typedef struct {
     uint32_t        variant;     /* row of bin_ranges */
     uint32_t        mode;        /* 0 aligned window, 1 exact-start window */
     float           distance;
-    uint32_t        pad0;
+    uint32_t        chunk_factor;  /* persistent launch: chunks per queue pull */
     void           *results;     /* resEntry_t[] in caller order */
+    uint32_t       *queue_heads; /* persistent launch: one atomic head per variant */
 } myers_verify_args_t;
 
 /* Variant table compiled into the library. */
@@ -50,10 +51,15 @@ int      myers_variant_desc(uint32_t v, myers_variant_desc_t *out);
 /* Workgroups of variant v resident per CU (occupancy), 0 on error. */
 int      myers_variant_occupancy(uint32_t v, int *wg_per_cu);
 
-/* Verify the slots of one variant.  grid = number of one-wave workgroups;
- * the kernel strides over chunks of `slots` candidates. */
+/* Verify the slots of one variant.  grid = number of one-wave workgroups.
+ * Plain launch: the grid strides over chunks of `slots` candidates.
+ * Persistent launch (needs bin_ranges and queue_heads): each workgroup pulls
+ * chunk_factor chunks at a time from the variant's queue head until the bin
+ * is drained. */
 int myers_launch_verify(uint32_t v, const myers_verify_args_t *args, uint32_t grid,
-                        hipStream_t stream);
+                        int persistent, hipStream_t stream);
+/* Resident workgroups per CU of the persistent kernel of variant v. */
+int myers_variant_occupancy_persistent(uint32_t v, int *wg_per_cu);
 
 /* Device-side length binning.  bin_of_words[w] (w = ceil(m/32), 0..max_words)
  * gives the variant of a query of w words.  counts/offsets/cursors hold
@@ -64,7 +70,7 @@ int myers_launch_verify(uint32_t v, const myers_verify_args_t *args, uint32_t gr
 int myers_launch_binning(const uint32_t *qinfo, uint32_t num_queries, const void *cands, uint32_t num_cands,
                          const uint8_t *bin_of_words, uint32_t max_words, uint32_t num_variants,
                          uint32_t *counts, uint32_t *cursors, uint32_t *ranges, uint32_t *order,
-                         uint32_t *err, void *results, hipStream_t stream);
+                         uint32_t *err, void *results, uint32_t *queue_heads, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -195,19 +195,35 @@ __device__ __forceinline__ uint32_t from_lower_lane(uint32_t v, uint32_t not_fir
 }
 
 template <int L, int W>
-__global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
-    using S = Shape<L, W>;
-    __shared__ uint4 lds[S::kLdsU4];
-    __shared__ uint2 qtab[S::kQSlots];
+struct Workspace {
+    uint4 lds[Shape<L, W>::kLdsU4];
+    uint2 qtab[Shape<L, W>::kQSlots];
+};
 
+template <int L, int W>
+__device__ __forceinline__ void zero_row_init(Workspace<L, W> &ws) {
+    using S = Shape<L, W>;
+    if constexpr (S::kRows > 5) {
+        // the all-zero row read by lanes before their first column
+        uint32_t *l32 = reinterpret_cast<uint32_t *>(ws.lds);
+        for (uint32_t i = threadIdx.x; i < (uint32_t)(S::kGroups * S::kRowU4 * 4); i += 64u) {
+            const uint32_t u4 = i >> 2, g = u4 / S::kRowU4, r = u4 % S::kRowU4;
+            l32[((g * S::kRows + kZeroRow) * S::kRowU4 + r) * 4u + (i & 3u)] = 0u;
+        }
+    }
+}
+
+// Verify chunks chunk_lo, chunk_lo + stride, ... < chunk_hi of the slot range
+// [start, start + count); a chunk is the kSlots candidates of one wave.
+template <int L, int W>
+__device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint32_t start, uint32_t count,
+                                              uint32_t chunk_lo, uint32_t chunk_hi, uint32_t stride,
+                                              Workspace<L, W> &ws) {
+    using S = Shape<L, W>;
+    uint4 *lds = ws.lds;
+    uint2 *qtab = ws.qtab;
     const uint32_t lane = threadIdx.x;
     const uint32_t gl = lane % L;                      // lane within the candidate's group
-    uint32_t start = a.start, count = a.count;
-    if (a.bin_ranges != nullptr) {
-        start = a.bin_ranges[2 * a.variant];
-        count = a.bin_ranges[2 * a.variant + 1];
-    }
-    const uint32_t nchunks = (count + S::kSlots - 1) / S::kSlots;
     const uint32_t *__restrict__ ref = a.ref;
     const int32_t last_word = (int32_t)(a.ref_words - 1);
     const CandRec *__restrict__ cands = static_cast<const CandRec *>(a.cands);
@@ -216,22 +232,7 @@ __global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
     const uint32_t not_first = gl != 0u ? ~0u : 0u;
     const uint64_t lanes_upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
 
-    if constexpr (S::kRows > 5) {
-        // the all-zero row read by lanes before their first column
-        uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
-        for (uint32_t i = lane; i < (uint32_t)(S::kGroups * S::kRowU4 * 4); i += 64u) {
-            const uint32_t u4 = i >> 2, g = u4 / S::kRowU4, r = u4 % S::kRowU4;
-            l32[((g * S::kRows + kZeroRow) * S::kRowU4 + r) * 4u + (i & 3u)] = 0u;
-        }
-    }
-
-    // Workgroups are dispatched round-robin over the 8 XCDs; renumber them so
-    // that each XCD (own L2) takes a contiguous run of chunks, which keeps the
-    // candidates of one query -- and their Peq -- in one L2.
-    uint32_t first = blockIdx.x;
-    if ((gridDim.x & 7u) == 0u) first = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
-
-    for (uint32_t chunk = first; chunk < nchunks; chunk += gridDim.x) {
+    for (uint32_t chunk = chunk_lo; chunk < chunk_hi; chunk += stride) {
         const uint32_t slot = chunk * S::kSlots + lane / L;
         const bool valid = slot < count;
         uint32_t cand = 0, m = 0, pos_entry = 0, query = 0xFFFFFFFFu;
@@ -371,6 +372,49 @@ __global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
     }
 }
 
+// Plain launch: a fixed grid strides over the chunks of one slot range.
+template <int L, int W>
+__global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
+    using S = Shape<L, W>;
+    __shared__ Workspace<L, W> ws;
+    uint32_t start = a.start, count = a.count;
+    if (a.bin_ranges != nullptr) {
+        start = a.bin_ranges[2 * a.variant];
+        count = a.bin_ranges[2 * a.variant + 1];
+    }
+    const uint32_t nchunks = (count + S::kSlots - 1) / S::kSlots;
+    zero_row_init<L, W>(ws);
+    // Workgroups are dispatched round-robin over the 8 XCDs; renumber them so
+    // that each XCD (own L2) takes a contiguous run of chunks, which keeps the
+    // candidates of one query -- and their Peq -- in one L2.
+    uint32_t first = blockIdx.x;
+    if ((gridDim.x & 7u) == 0u) first = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    verify_chunks<L, W>(a, start, count, first, nchunks, gridDim.x, ws);
+}
+
+// Persistent launch: a resident grid pulls work items of `chunk_factor`
+// chunks from the variant's device queue (an atomic head reset by the
+// binning) until the bin is drained; every wave reaches the exit test.
+template <int L, int W>
+__global__ __launch_bounds__(64) void verify_persistent_kernel(myers_verify_args_t a) {
+    using S = Shape<L, W>;
+    __shared__ Workspace<L, W> ws;
+    __shared__ uint32_t item_sh;
+    const uint32_t start = a.bin_ranges[2 * a.variant];
+    const uint32_t count = a.bin_ranges[2 * a.variant + 1];
+    const uint32_t nchunks = (count + S::kSlots - 1) / S::kSlots;
+    const uint32_t cf = a.chunk_factor ? a.chunk_factor : 1u;
+    zero_row_init<L, W>(ws);
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) item_sh = atomicAdd(&a.queue_heads[a.variant], cf);
+        __syncthreads();
+        const uint32_t lo = __builtin_amdgcn_readfirstlane(item_sh);
+        if (lo >= nchunks) break;
+        verify_chunks<L, W>(a, start, count, lo, min(lo + cf, nchunks), 1u, ws);
+    }
+}
+
 // ---------------------------------------------------------------- binning
 
 __global__ __launch_bounds__(64) void bin_reset_kernel(uint32_t *counts, uint32_t *err, uint32_t nv) {
@@ -411,15 +455,19 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const uint2 *qinfo, uint
 }
 
 __global__ __launch_bounds__(64) void bin_scan_kernel(const uint32_t *counts, uint32_t nv,
-                                                      uint32_t *ranges, uint32_t *cursors) {
+                                                      uint32_t *ranges, uint32_t *cursors,
+                                                      uint32_t *queue_heads) {
     if (threadIdx.x == 0) {
         uint32_t acc = 0;
         for (uint32_t v = 0; v < nv; v++) {
             ranges[2 * v] = acc;
             ranges[2 * v + 1] = counts[v];
             acc += counts[v];
-            cursors[v] = 0;
         }
+    }
+    for (uint32_t v = threadIdx.x; v < nv; v += blockDim.x) {
+        cursors[v] = 0;
+        queue_heads[v] = 0;
     }
 }
 
@@ -471,8 +519,11 @@ namespace {
 struct VariantEntry {
     uint32_t lanes, words;
     const void *fn;
+    const void *persistent_fn;
 };
-#define MYERS_VARIANT_ENTRY(L, W) {L, W, reinterpret_cast<const void *>(&verify_kernel<L, W>)},
+#define MYERS_VARIANT_ENTRY(L, W)                                                                \
+    {L, W, reinterpret_cast<const void *>(&verify_kernel<L, W>),                                \
+     reinterpret_cast<const void *>(&verify_persistent_kernel<L, W>)},
 const VariantEntry kVariants[] = {MYERS_VARIANT_LIST(MYERS_VARIANT_ENTRY)};
 constexpr uint32_t kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 static_assert(kNumVariants <= MYERS_MAX_VARIANTS, "variant table too large");
@@ -487,7 +538,8 @@ extern "C" int myers_variant_desc(uint32_t v, myers_variant_desc_t *out) {
     out->words = W;
     out->total_words = L * W;
     out->slots = 64u / L;
-    out->lds_bytes = ((W + 3u) / 4u) * 5u * 64u * 16u;
+    hipFuncAttributes attr;
+    out->lds_bytes = hipFuncGetAttributes(&attr, kVariants[v].fn) == hipSuccess ? (uint32_t)attr.sharedSizeBytes : 0u;
     return 0;
 }
 
@@ -500,11 +552,22 @@ extern "C" int myers_variant_occupancy(uint32_t v, int *wg_per_cu) {
     return 0;
 }
 
+extern "C" int myers_variant_occupancy_persistent(uint32_t v, int *wg_per_cu) {
+    if (v >= kNumVariants || wg_per_cu == nullptr) return -1;
+    int blocks = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kVariants[v].persistent_fn, 64, 0);
+    if (e != hipSuccess) return (int)e;
+    *wg_per_cu = blocks;
+    return 0;
+}
+
 extern "C" int myers_launch_verify(uint32_t v, const myers_verify_args_t *args, uint32_t grid,
-                                   hipStream_t stream) {
+                                   int persistent, hipStream_t stream) {
     if (v >= kNumVariants || args == nullptr || grid == 0) return -1;
+    if (persistent && (args->bin_ranges == nullptr || args->queue_heads == nullptr)) return -1;
     void *kargs[] = {const_cast<myers_verify_args_t *>(args)};
-    hipError_t e = hipLaunchKernel(kVariants[v].fn, dim3(grid), dim3(64), kargs, 0, stream);
+    hipError_t e = hipLaunchKernel(persistent ? kVariants[v].persistent_fn : kVariants[v].fn, dim3(grid),
+                                   dim3(64), kargs, 0, stream);
     return (int)e;
 }
 
@@ -513,7 +576,7 @@ extern "C" int myers_launch_binning(const uint32_t *qinfo, uint32_t num_queries,
                                     const uint8_t *bin_of_words, uint32_t max_words,
                                     uint32_t num_variants, uint32_t *counts, uint32_t *cursors,
                                     uint32_t *ranges, uint32_t *order, uint32_t *err,
-                                    void *results, hipStream_t stream) {
+                                    void *results, uint32_t *queue_heads, hipStream_t stream) {
     if (num_variants == 0 || num_variants > MYERS_MAX_VARIANTS) return -1;
     const uint2 *qi = reinterpret_cast<const uint2 *>(qinfo);
     const CandRec *cr = static_cast<const CandRec *>(cands);
@@ -524,7 +587,7 @@ extern "C" int myers_launch_binning(const uint32_t *qinfo, uint32_t num_queries,
     hipLaunchKernelGGL(bin_count_kernel, dim3(grid), dim3(256), 0, stream, qi, num_queries, cr, num_cands,
                        bin_of_words, max_words, num_variants, counts, err);
     hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(64), 0, stream, counts, num_variants, ranges,
-                       cursors);
+                       cursors, queue_heads);
     hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid), dim3(256), 0, stream, qi, num_queries, cr, num_cands,
                        bin_of_words, max_words, num_variants, ranges, cursors, order,
                        static_cast<uint2 *>(results));

@@ -1,0 +1,128 @@
+#!/usr/bin/env python3
+"""Autotuner for the length-bin table of libmyers_amd.so.
+
+For each representative read length m (query words w = ceil(m/32)) it times
+every kernel variant whose bit-vector covers w without more than ~2x padding
+(lanes per read L, u32 words per lane W), each with several chunk factors
+(chunks of 64/L candidates a persistent workgroup takes per queue pull, i.e.
+candidates per workgroup pull = chunk_factor * 64 / L), on a synthetic batch
+of reads of that length, and keeps the fastest.  The result is written as the
+library's tuning table (lines "maxWords lanes words chunkFactor"), which
+initMyers loads from tuning/gfx950.tsv next to the library directory.
+
+  python myers-gpu_amd/python/autotune.py [--out myers-gpu_amd/tuning/gfx950.tsv]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import myers_amd as M  # noqa: E402
+
+SHORT = [64, 100, 128, 160, 200, 256, 320, 400, 480, 600, 700, 800, 900, 1000]
+LONG = [1500, 2000, 3000, 4000, 5000, 6500, 8000, 10000, 13000, 16000, 20000, 24576]
+CHUNK_FACTORS = (1, 2, 4)
+REF_SIZE = 200_000_000
+TARGET_CELLS = 1.2e11
+
+
+def log(msg):
+    print(f"[autotune] {msg}", file=sys.stderr, flush=True)
+
+
+def candidates_for(variants, w):
+    out = []
+    for i, v in enumerate(variants):
+        t = v["lanes"] * v["words"]
+        if t >= w and t <= max(2 * w, w + 4):
+            out.append(i)
+    return out
+
+
+def time_variant(m, bufs, b, v, cf, reps):
+    table = m.bin_table()
+    w = (int(b["qinfo"][0, 1]) + 31) // 32
+    table[w] = v
+    m.set_bin_table(table)
+    m.set_chunk_factor(v, cf)
+    dp, dq, dc, dr = bufs
+    nq, nc, ne = len(b["qinfo"]), len(b["cands"]), len(b["peq"])
+    M.verify_device(m, dp, ne, dq, nq, dc, nc, 0.1, dr, flags=M.ENGINE_PERSISTENT, h_qinfo=b["qinfo"])
+    M.synchronize(m)
+    best = float("inf")
+    for _ in range(reps):
+        M.verify_device(m, dp, ne, dq, nq, dc, nc, 0.1, dr, flags=M.ENGINE_PERSISTENT, h_qinfo=b["qinfo"])
+        M.synchronize(m)
+        best = min(best, m.stats().kernelMs)
+    return best
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=os.path.join(M.PKG_DIR, "tuning", "gfx950.tsv"))
+    ap.add_argument("--json", default=None, help="write every measurement here")
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--lengths", default=None, help="comma-separated read lengths (default: built-in set)")
+    args = ap.parse_args()
+    lengths = [int(x) for x in args.lengths.split(",")] if args.lengths else SHORT + LONG
+
+    ref = M.synth_reference(99, REF_SIZE)
+    # start from the built-in table (no tuning file) so earlier tables do not leak in
+    os.environ["MYERS_TUNING_FILE"] = os.devnull
+    m = M.Myers(ref, REF_SIZE, avg_query=1000, cands_per_query=11, buffer_bytes=1 << 30)
+    variants = m.variants()
+    default_table = m.bin_table()
+    records, chosen = [], []
+    for length in lengths:
+        w = (length + 31) // 32
+        cands_per_read = 11 if length <= 1024 else 4
+        n_cands = int(TARGET_CELLS / (length * 1.2 * length))
+        # enough candidates to fill 256 CUs even at 64 lanes per read
+        n_reads = max(-(-4096 // cands_per_read), min(200_000, n_cands // cands_per_read))
+        spec = M.make_spec(1000 + length, n_reads, length=length, sub=0.03, ins=0.005, dele=0.005,
+                           candidates=cands_per_read, distance=0.1)
+        b = M.synth_batch(spec, ref, REF_SIZE)
+        bufs = (M.DeviceArray(b["peq"]), M.DeviceArray(b["qinfo"]), M.DeviceArray(b["cands"]),
+                M.DeviceArray.empty((len(b["cands"]), 2), np.uint32))
+        best = None
+        for v in candidates_for(variants, w):
+            for cf in CHUNK_FACTORS:
+                ms = time_variant(m, bufs, b, v, cf, args.reps)
+                gcups = b["cells"] / (ms / 1e3) / 1e9
+                rec = {"length": length, "words": w, "lanes": variants[v]["lanes"], "wpl": variants[v]["words"],
+                       "chunk_factor": cf, "kernel_ms": ms, "gcups": gcups, "candidates": len(b["cands"])}
+                records.append(rec)
+                if best is None or ms < best[0]:
+                    best = (ms, v, cf, gcups)
+        for x in bufs:
+            x.free()
+        m.set_bin_table(default_table)
+        ms, v, cf, gcups = best
+        chosen.append((w, variants[v]["lanes"], variants[v]["words"], cf, length, gcups))
+        log(f"m={length:5d} w={w:3d}: L{variants[v]['lanes']}xW{variants[v]['words']} cf={cf} "
+            f"{gcups:9.0f} GCUPS (default L{variants[default_table[w]]['lanes']}xW{variants[default_table[w]]['words']})")
+    m.close()
+
+    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    with open(args.out, "w") as f:
+        f.write("# maxWords lanes words chunkFactor  -- written by autotune.py on gfx950\n")
+        f.write(f"# {time.strftime('%Y-%m-%d')}: best persistent-kernel time per read length "
+                "(reads of m bases use the first line with maxWords >= ceil(m/32))\n")
+        prev = 0
+        for w, lanes, words, cf, length, gcups in sorted(chosen):
+            if w <= prev:
+                continue
+            f.write(f"{w} {lanes} {words} {cf}    # m={length}: {gcups:.0f} GCUPS\n")
+            prev = w
+    log(f"wrote {args.out}")
+    if args.json:
+        with open(args.json, "w") as f:
+            json.dump(records, f)
+
+
+if __name__ == "__main__":
+    main()

@@ -28,6 +28,7 @@ WINDOW_EXACT = 1
 ENGINE_AUTO = 0
 ENGINE_PLAIN = 1 << 4
 ENGINE_PERSISTENT = 2 << 4
+USE_GRAPH = 1 << 6
 
 ERRORS = {
     0: "SUCCESS", 1: "E_OPENING_FILE", 2: "E_READING_FILE", 3: "E_INSUFFICIENT_MEM_GPU",
@@ -54,13 +55,32 @@ class Stats(ctypes.Structure):
                 ("kernelMs", ctypes.c_double), ("batchMs", ctypes.c_double),
                 ("cells", ctypes.c_uint64), ("candidates", ctypes.c_uint64),
                 ("kernelMsTotal", ctypes.c_double), ("kernelLaunchesTotal", ctypes.c_uint64),
-                ("batchesTotal", ctypes.c_uint64)]
+                ("batchesTotal", ctypes.c_uint64), ("graphCaptures", ctypes.c_uint64)]
 
 
 class VariantInfo(ctypes.Structure):
     _fields_ = [("lanes", ctypes.c_uint32), ("words", ctypes.c_uint32),
                 ("maxQuerySize", ctypes.c_uint32), ("candidatesPerWave", ctypes.c_uint32),
                 ("ldsBytes", ctypes.c_uint32), ("workgroupsPerCU", ctypes.c_int32)]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("structSize", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("distance", ctypes.c_float),
+                ("flags", ctypes.c_uint32), ("hbmBudgetBytes", ctypes.c_uint64), ("pollMicros", ctypes.c_uint32),
+                ("maxBatchCandidates", ctypes.c_uint32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("peq", ctypes.c_void_p), ("qinfo", ctypes.c_void_p), ("cands", ctypes.c_void_p),
+                ("results", ctypes.c_void_p), ("numPEQEntries", ctypes.c_uint32),
+                ("numQueries", ctypes.c_uint32), ("numCandidates", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("structSize", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("requests", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("candidates", ctypes.c_uint64),
+                ("maxBatchRequests", ctypes.c_uint64), ("busyMs", ctypes.c_double)]
 
 
 class SynthSpec(ctypes.Structure):
@@ -144,6 +164,18 @@ def lib():
         L.myersGetVariantInfo.argtypes = [vp, u32, ctypes.POINTER(VariantInfo)]
         L.myersGetBinTable.argtypes = [vp, vp, u32]
         L.myersSetBinTable.argtypes = [vp, vp, u32]
+        L.myersSetChunkFactor.argtypes = [vp, u32, u32]
+        L.myersGetChunkFactor.argtypes = [vp, u32, ctypes.POINTER(ctypes.c_uint32)]
+        L.myersLoadTuning.argtypes = [vp, ctypes.c_char_p]
+        L.myersServeDefaultConfig.argtypes = [ctypes.POINTER(ServeConfig)]
+        L.myersServeDefaultConfig.restype = None
+        L.myersServeStart.argtypes = [pvp, pvp, u32, ctypes.POINTER(ServeConfig)]
+        L.myersServeSubmit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]
+        L.myersServeWait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+        L.myersServeGetStats.argtypes = [vp, ctypes.POINTER(ServeStats)]
+        L.myersServeStop.argtypes = [vp]
+        L.myersServeRunOpenLoop.argtypes = [vp, u32, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64),
+                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.myersDefaultConfig.argtypes = [ctypes.POINTER(Config)]
         L.myersDefaultConfig.restype = None
         _lib = L
@@ -351,6 +383,23 @@ class Myers:
         check(lib().myersGetBinTable(self.buffer(0), _ptr(t), max_words), "myersGetBinTable")
         return t
 
+    def set_chunk_factor(self, variant, chunks):
+        check(lib().myersSetChunkFactor(self.buffer(0), variant, chunks), "myersSetChunkFactor")
+
+    def chunk_factor(self, variant):
+        c = ctypes.c_uint32()
+        check(lib().myersGetChunkFactor(self.buffer(0), variant, ctypes.byref(c)), "myersGetChunkFactor")
+        return c.value
+
+    def load_tuning(self, path):
+        check(lib().myersLoadTuning(self.buffer(0), path.encode()), "myersLoadTuning")
+
+    def variant_index(self, lanes, words):
+        for i, d in enumerate(self.variants()):
+            if d["lanes"] == lanes and d["words"] == words:
+                return i
+        raise KeyError((lanes, words))
+
     def set_bin_table(self, table):
         t = np.ascontiguousarray(table, dtype=np.uint8)
         check(lib().myersSetBinTable(self.buffer(0), _ptr(t), len(t) - 1), "myersSetBinTable")
@@ -405,3 +454,72 @@ def verify_device(m, peq_d, num_entries, qinfo_d, num_queries, cands_d, num_cand
 
 def synchronize(m, i=0):
     check(lib().myersSynchronize(m.buffer(i)), "myersSynchronize")
+
+
+# ----------------------------------------------------------------- serving
+
+class Server:
+    """Continuous-batching server over all buffers of a Myers instance."""
+
+    def __init__(self, myers, distance=0.1, flags=0, hbm_budget=0, poll_us=0, max_batch_candidates=0):
+        cfg = ServeConfig()
+        lib().myersServeDefaultConfig(ctypes.byref(cfg))
+        cfg.distance = distance
+        cfg.flags = flags
+        if hbm_budget:
+            cfg.hbmBudgetBytes = hbm_budget
+        if poll_us:
+            cfg.pollMicros = poll_us
+        cfg.maxBatchCandidates = max_batch_candidates
+        self._h = ctypes.c_void_p()
+        check(lib().myersServeStart(ctypes.byref(self._h), myers._handles, myers.num_buffers, ctypes.byref(cfg)),
+              "myersServeStart")
+        self._keep = []
+
+    @staticmethod
+    def request(peq, qinfo, cands):
+        """A Request over numpy arrays (kept alive by the returned tuple)."""
+        peq = np.ascontiguousarray(peq, dtype=np.uint32)
+        qinfo = np.ascontiguousarray(qinfo, dtype=np.uint32)
+        cands = np.ascontiguousarray(cands, dtype=CAND_DTYPE)
+        res = np.zeros((len(cands), 2), dtype=np.uint32)
+        r = Request(peq.ctypes.data, qinfo.ctypes.data, cands.ctypes.data, res.ctypes.data,
+                    len(peq), len(qinfo), len(cands), 0)
+        return r, (peq, qinfo, cands, res)
+
+    def submit(self, req):
+        t = ctypes.c_uint64()
+        check(lib().myersServeSubmit(self._h, ctypes.byref(req), ctypes.byref(t)), "myersServeSubmit")
+        return t.value
+
+    def wait(self, ticket):
+        lat = ctypes.c_double()
+        check(lib().myersServeWait(self._h, ticket, ctypes.byref(lat)), "myersServeWait")
+        return lat.value
+
+    def run_open_loop(self, reqs, arrivals_ns):
+        n = len(reqs)
+        arr = (Request * n)(*reqs)
+        times = np.ascontiguousarray(arrivals_ns, dtype=np.uint64)
+        lat = np.zeros(n, dtype=np.float64)
+        wall = ctypes.c_double()
+        check(lib().myersServeRunOpenLoop(self._h, n, arr, times.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                          lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(wall)),
+              "myersServeRunOpenLoop")
+        return lat, wall.value
+
+    def stats(self):
+        s = ServeStats()
+        check(lib().myersServeGetStats(self._h, ctypes.byref(s)), "myersServeGetStats")
+        return s
+
+    def stop(self):
+        if self._h:
+            check(lib().myersServeStop(self._h), "myersServeStop")
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.stop()

@@ -219,3 +219,57 @@ def test_repeat_sends_are_independent():
         m.verify(b["peq"], b["qinfo"], b["cands"][:half], 0.1)
         again = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1)
     np.testing.assert_array_equal(first, again)
+
+
+@pytest.mark.parametrize("engine", [M.ENGINE_PLAIN, M.ENGINE_PERSISTENT])
+def test_engines_mixed_lengths(engine):
+    """Plain per-bin launches and persistent queue-draining launches give identical results."""
+    announce(f"engine {engine}")
+    ref, b = synth_case(31, 3_000_000, 900, choices=[64, 100, 200, 333, 600, 1000, 1500, 3000],
+                        err_range=(0.01, 0.08), sub_frac=0.7, poisson=9, cap=40, distance=0.1)
+    with M.Myers(ref, 3_000_000, avg_query=800, cands_per_query=9, buffer_bytes=256 << 20) as m:
+        for mode in (M.WINDOW_ALIGNED, M.WINDOW_EXACT):
+            got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=mode | engine)
+            assert m.stats().binned == 1
+            np.testing.assert_array_equal(got, oracle_for(ref, 3_000_000, b, 0.1, mode))
+
+
+def test_persistent_single_length():
+    """The persistent engine on a one-bin batch (binned, one queue)."""
+    announce("persistent single bin")
+    ref, b = synth_case(32, 1_000_000, 700, length=1000, sub=0.03, ins=0.005, dele=0.005, candidates=11,
+                        distance=0.1)
+    with M.Myers(ref, 1_000_000, avg_query=1000, cands_per_query=11, buffer_bytes=256 << 20) as m:
+        got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=M.ENGINE_PERSISTENT)
+    np.testing.assert_array_equal(got, oracle_for(ref, 1_000_000, b, 0.1, 0))
+
+
+@pytest.mark.parametrize("engine", [M.ENGINE_PLAIN, M.ENGINE_PERSISTENT])
+def test_graph_replays_match_eager(engine):
+    """A captured pipeline, replayed several times with eager work and
+    synchronisations in between, matches the eager result every time."""
+    announce(f"graph replays engine {engine}")
+    ref, b = synth_case(33, 2_000_000, 800, choices=[100, 400, 1000], err_range=(0.01, 0.05), poisson=11,
+                        distance=0.1)
+    ref2, b2 = synth_case(34, 2_000_000, 300, length=150, sub=0.03, ins=0.005, dele=0.005, candidates=5,
+                          distance=0.1)
+    with M.Myers(ref, 2_000_000, avg_query=500, cands_per_query=11, buffer_bytes=256 << 20) as m:
+        dp, dq, dc = M.DeviceArray(b["peq"]), M.DeviceArray(b["qinfo"]), M.DeviceArray(b["cands"])
+        dr = M.DeviceArray.empty((len(b["cands"]), 2), np.uint32)
+        nq, nc, ne = len(b["qinfo"]), len(b["cands"]), len(b["peq"])
+        M.verify_device(m, dp, ne, dq, nq, dc, nc, 0.1, dr, flags=engine, h_qinfo=b["qinfo"])
+        M.synchronize(m)
+        eager = dr.get()
+        np.testing.assert_array_equal(eager, oracle_for(ref, 2_000_000, b, 0.1, 0))
+        for rep in range(4):
+            dr_clear = np.zeros((nc, 2), np.uint32)
+            M.check(M.lib().myersMemcpyHtoD(dr.ptr, M._ptr(dr_clear), dr_clear.nbytes), "clear")
+            M.verify_device(m, dp, ne, dq, nq, dc, nc, 0.1, dr, flags=engine | M.USE_GRAPH, h_qinfo=b["qinfo"])
+            M.synchronize(m)
+            np.testing.assert_array_equal(dr.get(), eager, err_msg=f"replay {rep}")
+            # eager work of another shape on the same buffer between replays
+            other = m.verify(b2["peq"], b2["qinfo"], b2["cands"], 0.1)
+            assert len(other) == len(b2["cands"])
+        assert m.stats().graphCaptures == 1
+        for x in (dp, dq, dc, dr):
+            x.free()

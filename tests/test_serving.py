@@ -1,0 +1,117 @@
+"""Continuous-batching server: every request's results are the same alone and
+in the mix, and equal to the oracle; admission honours the HBM budget."""
+import numpy as np
+import pytest
+
+import myers_amd as M
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+REF_SIZE = 3_000_000
+
+
+def announce(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def carve_requests(b, sizes):
+    """Split a synthetic batch into requests of `sizes` reads each (rebased arrays)."""
+    reqs, q0 = [], 0
+    per_read = np.bincount(b["cands"]["query"], minlength=len(b["qinfo"]))
+    cand_start = np.concatenate([[0], np.cumsum(per_read)])
+    for n in sizes:
+        q1 = min(q0 + n, len(b["qinfo"]))
+        qi = b["qinfo"][q0:q1].copy()
+        e0 = qi[0, 0]
+        e1 = qi[-1, 0] + (qi[-1, 1] + 127) // 128
+        peq = b["peq"][e0:e1]
+        qi[:, 0] -= e0
+        c = b["cands"][cand_start[q0]:cand_start[q1]].copy()
+        c["query"] -= q0
+        reqs.append((peq, qi, c))
+        q0 = q1
+    return reqs
+
+
+def make_pool(seed):
+    ref = M.synth_reference(seed, REF_SIZE)
+    short = M.synth_batch(M.make_spec(seed + 1, 1200, choices=[100, 200, 400, 600, 800, 1000],
+                                      err_range=(0.01, 0.05), sub_frac=0.8, poisson=11, cap=64, distance=0.1),
+                          ref, REF_SIZE)
+    long = M.synth_batch(M.make_spec(seed + 2, 60, log_uniform=(2000, 20000), err_range=(0.12, 0.12),
+                                     sub_frac=0.5, candidates=4, distance=0.1), ref, REF_SIZE)
+    rng = np.random.default_rng(seed)
+    sizes = [int(x) for x in np.exp(rng.uniform(0, np.log(200), size=24)).round()]
+    reqs = carve_requests(short, sizes)
+    reqs += carve_requests(long, [1, 3, 5, 2, 10])
+    order = rng.permutation(len(reqs))
+    return ref, [reqs[i] for i in order]
+
+
+def oracle_req(ref, req):
+    peq, qi, c = req
+    return O.myers_batch(ref, REF_SIZE, peq, qi, c, 0.1, O.MODE_ALIGNED)
+
+
+def test_results_same_alone_and_in_the_mix():
+    announce("serve mix invariance")
+    ref, reqs = make_pool(41)
+    with M.Myers(ref, REF_SIZE, num_buffers=3, avg_query=800, cands_per_query=11, buffer_bytes=256 << 20) as m:
+        alone = [m.verify(*r, 0.1) for r in reqs]
+        with M.Server(m, distance=0.1) as sv:
+            made = [M.Server.request(*r) for r in reqs]
+            tickets = [sv.submit(x[0]) for x in made]
+            for t in tickets:
+                assert sv.wait(t) >= 0.0
+            st = sv.stats()
+            assert st.requests == len(reqs)
+            assert st.maxBatchRequests >= 2          # requests really shared batches
+        for (req, keep), a, r in zip(made, alone, reqs):
+            np.testing.assert_array_equal(keep[3], a)
+            np.testing.assert_array_equal(a, oracle_req(ref, r))
+
+
+def test_open_loop_poisson():
+    announce("serve open loop")
+    ref, reqs = make_pool(42)
+    rng = np.random.default_rng(5)
+    with M.Myers(ref, REF_SIZE, num_buffers=3, avg_query=800, cands_per_query=11, buffer_bytes=256 << 20) as m:
+        with M.Server(m, distance=0.1) as sv:
+            made = [M.Server.request(*r) for r in reqs * 3]
+            gaps = rng.exponential(1e6 / 2000.0, size=len(made))          # 2000 requests/s
+            arrivals = (np.cumsum(gaps) * 1e3).astype(np.uint64)
+            lat, wall = sv.run_open_loop([x[0] for x in made], arrivals)
+        assert np.all(lat >= 0) and wall > 0
+        for (req, keep), r in zip(made, reqs * 3):
+            np.testing.assert_array_equal(keep[3], oracle_req(ref, r))
+
+
+def test_budget_limits_admission():
+    announce("serve budget")
+    ref, reqs = make_pool(43)
+    with M.Myers(ref, REF_SIZE, num_buffers=2, avg_query=800, cands_per_query=11, buffer_bytes=256 << 20) as m:
+        with M.Server(m, distance=0.1, hbm_budget=1) as sv:     # one request in flight at a time
+            made = [M.Server.request(*r) for r in reqs[:10]]
+            tickets = [sv.submit(x[0]) for x in made]
+            for t in tickets:
+                sv.wait(t)
+            assert sv.stats().maxBatchRequests == 1
+        for (req, keep), r in zip(made, reqs[:10]):
+            np.testing.assert_array_equal(keep[3], oracle_req(ref, r))
+
+
+def test_oversized_request_rejected():
+    announce("serve oversized")
+    ref, reqs = make_pool(44)
+    with M.Myers(ref, REF_SIZE, num_buffers=1, avg_query=100, cands_per_query=1,
+                 buffer_bytes=1 << 16) as m:
+        cap = m.capacity()[2]
+        big = max(reqs, key=lambda r: len(r[2]))
+        assert len(big[2]) > cap
+        with M.Server(m, distance=0.1) as sv:
+            req, keep = M.Server.request(*big)
+            t = sv.submit(req)
+            with pytest.raises(M.MyersError) as e:
+                sv.wait(t)
+            assert e.value.code == 17

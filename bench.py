@@ -122,6 +122,116 @@ def _cpu_model():
     return "unknown"
 
 
+def run_steps(m, bufs, b, flags, steps, warmup):
+    """Time `steps` passes of the pipeline over a device-resident batch."""
+    dp, dq, dc, dr = bufs
+    nq, nc = len(b["qinfo"]), len(b["cands"])
+
+    def step():
+        M.verify_device(m, dp, len(b["peq"]), dq, nq, dc, nc, DISTANCE, dr, flags=flags, h_qinfo=b["qinfo"])
+
+    for _ in range(warmup):
+        step()
+    M.synchronize(m)
+    M.lib().myersResetStats(m.buffer(0))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    M.synchronize(m)     # waits for the last step; every step's events are folded in
+    t1 = time.perf_counter()
+    st = m.stats()
+    if st.batchesTotal != steps:
+        raise SystemExit(f"bench: {st.batchesTotal} timed batches for {steps} steps")
+    return {"elapsed": t1 - t0, "kernel_ms": st.kernelMsTotal / st.batchesTotal,
+            "launches": st.kernelLaunchesTotal / st.batchesTotal, "binned": bool(st.binned),
+            "cand_per_s": nc * steps / (t1 - t0)}
+
+
+# ------------------------------------------------------------------ serving
+
+def serving_pool(ref, ref_size, seed, n_requests):
+    """Config-5 requests: 1..4096 reads each (log-uniform); 90% of reads from the
+    mixed-length set (config 3), 10% long noisy reads (config 4)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(0.0, np.log(4096.0), size=n_requests)).round().astype(int).clip(1, 4096)
+    n_long = [int(rng.binomial(s, 0.1)) for s in sizes]
+    short = M.synth_batch(M.make_spec(seed + 1, int(sum(sizes) - sum(n_long)) + 1, distance=DISTANCE,
+                                      **CONFIGS[3]["spec"]), ref, ref_size)
+    long = M.synth_batch(M.make_spec(seed + 2, max(1, sum(n_long)), distance=DISTANCE, **CONFIGS[4]["spec"]),
+                         ref, ref_size)
+
+    def index(b):
+        per = np.bincount(b["cands"]["query"], minlength=len(b["qinfo"]))
+        return np.concatenate([[0], np.cumsum(per)])
+
+    cs, cl = index(short), index(long)
+    reqs = []
+    si = li = 0
+    for s, nl in zip(sizes, n_long):
+        parts = [(short, cs, si, si + s - nl), (long, cl, li, li + nl)]
+        si += s - nl
+        li += nl
+        peqs, qis, cands, e_off, q_off = [], [], [], 0, 0
+        for b, cix, q0, q1 in parts:
+            if q1 <= q0:
+                continue
+            qi = b["qinfo"][q0:q1].copy()
+            e0 = qi[0, 0]
+            e1 = qi[-1, 0] + (qi[-1, 1] + 127) // 128
+            peqs.append(b["peq"][e0:e1])
+            qi[:, 0] = (qi[:, 0].astype(np.int64) - int(e0) + e_off).astype(np.uint32)
+            qis.append(qi)
+            c = b["cands"][cix[q0]:cix[q1]].copy()
+            c["query"] = (c["query"].astype(np.int64) - q0 + q_off).astype(np.uint32)
+            cands.append(c)
+            e_off += e1 - e0
+            q_off += q1 - q0
+        reqs.append((np.concatenate(peqs), np.concatenate(qis), np.concatenate(cands)))
+    return reqs
+
+
+def measure_serving(m, ref, ref_size, loads, seconds, seed=5):
+    pool = serving_pool(ref, ref_size, seed, 192)
+    cells = [int((r[1][r[2]["query"], 1].astype(np.int64) * window_len(r[1][r[2]["query"], 1])).sum()) for r in pool]
+    cands = [len(r[2]) for r in pool]
+    alone = [m.verify(*r, DISTANCE) for r in pool[:24]]          # reference results for the mix check
+    out = {"pool_requests": len(pool), "mean_reads_per_request": float(np.mean([len(r[1]) for r in pool])),
+           "mean_candidates_per_request": float(np.mean(cands)), "levels": []}
+    with M.Server(m, distance=DISTANCE) as sv:
+        # saturated throughput: every pool request arrives at once
+        made = [M.Server.request(*r) for r in pool]
+        lat, wall = sv.run_open_loop([x[0] for x in made], np.zeros(len(made), dtype=np.uint64))
+        sat_cps = sum(cands) / (wall / 1e3)
+        sat_rps = len(pool) / (wall / 1e3)
+        out["saturated_candidates_per_s"] = sat_cps
+        out["saturated_gcups"] = sum(cells) / (wall / 1e3) / 1e9
+        for (req, keep), a in zip(made[:24], alone):
+            if not np.array_equal(keep[3], a):
+                raise SystemExit("bench: served results differ from the same requests verified alone")
+        rng = np.random.default_rng(seed + 1)
+        for load in loads:
+            rate = load * sat_rps
+            n = max(50, int(rate * seconds))
+            idx = rng.integers(0, len(pool), size=n)
+            arrivals = (np.cumsum(rng.exponential(1.0 / rate, size=n)) * 1e9).astype(np.uint64)
+            made = [M.Server.request(*pool[i]) for i in idx]
+            lat, wall = sv.run_open_loop([x[0] for x in made], arrivals)
+            for (req, keep), i in zip(made, idx):
+                if i < len(alone) and not np.array_equal(keep[3], alone[i]):
+                    raise SystemExit("bench: served results differ from the same requests verified alone")
+            nc = sum(cands[i] for i in idx)
+            out["levels"].append({
+                "load": load, "requests": n, "offered_requests_per_s": rate,
+                "candidates_per_s": nc / (wall / 1e3), "gcups": sum(cells[i] for i in idx) / (wall / 1e3) / 1e9,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "p999_ms": float(np.percentile(lat, 99.9)), "max_ms": float(lat.max()),
+                "max_reads_per_request": int(max(len(pool[i][1]) for i in idx)),
+                "batches": int(sv.stats().batches)})
+            log(f"serving load {load:.2f}: {out['levels'][-1]['candidates_per_s']:.3e} cand/s, "
+                f"p50 {out['levels'][-1]['p50_ms']:.2f} ms, p99 {out['levels'][-1]['p99_ms']:.2f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,7 +240,11 @@ def main():
     ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
     ap.add_argument("--engine", default="auto", choices=["auto", "plain", "persistent"])
     ap.add_argument("--mode", default="aligned", choices=["aligned", "exact"])
+    ap.add_argument("--graph", action="store_true", help="replay the batch pipeline as a captured HIP graph")
+    ap.add_argument("--compare-engines", action="store_true", help="also time the other engine on the same batch")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
+    ap.add_argument("--serve-loads", default="0.7", help="serving load levels (fractions of saturation); '' = skip")
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--variant", default=None, help="force a kernel variant for every read length, e.g. L2xW16")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -148,44 +262,44 @@ def main():
     nc, nq = len(b["cands"]), len(b["qinfo"])
     log(f"batch: {nq} reads, {nc} candidates, {b['cells']:.3e} cells in {time.time() - t0:.1f}s")
 
-    flags = (M.WINDOW_EXACT if args.mode == "exact" else M.WINDOW_ALIGNED) | \
-        {"auto": M.ENGINE_AUTO, "plain": M.ENGINE_PLAIN, "persistent": M.ENGINE_PERSISTENT}[args.engine]
+    engines = {"auto": M.ENGINE_AUTO, "plain": M.ENGINE_PLAIN, "persistent": M.ENGINE_PERSISTENT}
+    mode_flag = M.WINDOW_EXACT if args.mode == "exact" else M.WINDOW_ALIGNED
+    flags = mode_flag | engines[args.engine] | (M.USE_GRAPH if args.graph else 0)
     avg = int(b["qinfo"][:, 1].mean())
-    m = M.Myers(ref, cfg["ref_size"], avg_query=avg, cands_per_query=max(1, nc // nq),
-                buffer_bytes=max(256 << 20, 2 * (b["peq"].nbytes + b["cands"].nbytes + nc * 12)))
+    serve_loads = [float(x) for x in args.serve_loads.split(",") if x.strip()]
+    num_buffers = 4 if serve_loads else 1
+    need = 2 * (b["peq"].nbytes + b["cands"].nbytes + nc * 12)
+    m = M.Myers(ref, cfg["ref_size"], num_buffers=num_buffers, avg_query=avg, cands_per_query=max(1, nc // nq),
+                buffer_bytes=max(512 << 20, need))
     if args.variant:
         lanes, words = (int(x) for x in args.variant.lstrip("L").split("xW"))
-        info = m.variants()
-        v = [i for i, d in enumerate(info) if d["lanes"] == lanes and d["words"] == words]
-        if not v:
-            raise SystemExit(f"bench: no variant {args.variant}")
+        v = m.variant_index(lanes, words)
         table = m.bin_table()
         w = np.arange(len(table))
-        table[(w >= 1) & (w <= lanes * words)] = v[0]
+        table[(w >= 1) & (w <= lanes * words)] = v
         m.set_bin_table(table)
-    dp, dq, dc = M.DeviceArray(b["peq"]), M.DeviceArray(b["qinfo"]), M.DeviceArray(b["cands"])
-    dr = M.DeviceArray.empty((nc, 2), np.uint32)
+    bufs = (M.DeviceArray(b["peq"]), M.DeviceArray(b["qinfo"]), M.DeviceArray(b["cands"]),
+            M.DeviceArray.empty((nc, 2), np.uint32))
 
-    def step():
-        M.verify_device(m, dp, len(b["peq"]), dq, nq, dc, nc, DISTANCE, dr, flags=flags, h_qinfo=b["qinfo"])
+    run = run_steps(m, bufs, b, flags, args.steps, args.warmup)
+    results = bufs[3].get()
+    comparison = None
+    if args.compare_engines:
+        other = {}
+        for name in ("plain", "persistent"):
+            r = run_steps(m, bufs, b, mode_flag | engines[name] | (M.USE_GRAPH if args.graph else 0),
+                          max(3, args.steps // 2), 1)
+            if not np.array_equal(bufs[3].get(), results):
+                raise SystemExit(f"bench: engine {name} differs from the timed run")
+            other[name] = r
+        comparison = {"plain_candidates_per_s": other["plain"]["cand_per_s"],
+                      "persistent_candidates_per_s": other["persistent"]["cand_per_s"],
+                      "plain_kernel_ms": other["plain"]["kernel_ms"],
+                      "persistent_kernel_ms": other["persistent"]["kernel_ms"],
+                      "persistent_over_plain": other["persistent"]["cand_per_s"] / other["plain"]["cand_per_s"]}
+        log(f"engines: {comparison}")
 
-    for _ in range(args.warmup):
-        step()
-    M.synchronize(m)
-    M.lib().myersResetStats(m.buffer(0))
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    M.synchronize(m)     # each step's completion is waited on by the next synchronize
-    t1 = time.perf_counter()
-    st = m.stats()
-    # The library records HIP events on the buffer's stream around each
-    # step's verification launches; myersSynchronize folds them in.
-    elapsed = t1 - t0
-    results = dr.get()
-    if st.batchesTotal != args.steps:
-        raise SystemExit(f"bench: {st.batchesTotal} timed batches for {args.steps} steps")
-    kernel_ms = st.kernelMsTotal / st.batchesTotal     # mean kernel time per step
+    elapsed, kernel_ms = run["elapsed"], run["kernel_ms"]
     value = nc * args.steps / elapsed
     cells = b["cells"]
     gcups = cells * args.steps / elapsed / 1e9
@@ -202,6 +316,11 @@ def main():
     if args.cpu_seconds > 0:
         log("CPU baseline / parity sample ...")
         cpu = cpu_baseline(ref, cfg["ref_size"], b, results, args.cpu_seconds)
+
+    serving = None
+    if serve_loads:
+        log("serving ...")
+        serving = measure_serving(m, ref, cfg["ref_size"], serve_loads, args.serve_seconds)
 
     out = {
         "metric": "candidates/s & GCUPS on 1 MI355X (bit-exact); p99 latency under serving load",
@@ -221,8 +340,8 @@ def main():
             "workload": f"{cfg['name']}: seed-{cfg['ref_seed']} {cfg['ref_size'] / 1e9:.3g} Gbase reference, "
                         f"{nq} reads, {nc} candidates, window = len*1.2, {args.mode} windows",
             "reads": nq, "candidates": nc, "cells": cells, "reference_bases": cfg["ref_size"],
-            "engine": args.engine, "variants": variants, "launches_per_step": st.launches,
-            "binned": bool(st.binned),
+            "engine": args.engine, "graph": args.graph, "variants": variants,
+            "launches_per_step": run["launches"], "binned": run["binned"],
         },
         "roofline": {
             "bound": "valu",
@@ -233,16 +352,18 @@ def main():
             "traffic": MEASURED_TRAFFIC.get((args.config, args.mode)),
             "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "kernel_ms": kernel_ms,
-            "kernel_launches_per_step": st.kernelLaunchesTotal / st.batchesTotal,
+            "kernel_launches_per_step": run["launches"],
             "ops_per_cell": K_GFX950 / 32.0,
             "frac_reference_ops": cells * K_REF / 32.0 / kernel_s / PEAK_INT_OPS,
             "hbm": {"achieved": hbm_achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": hbm_achieved / PEAK_HBM_GBS, "algorithmic_bytes": hbm_bytes},
         },
+        "engine_comparison": comparison,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
-    for a in (dp, dq, dc, dr):
+    for a in bufs:
         a.free()
     m.close()
 

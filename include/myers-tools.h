@@ -96,6 +96,35 @@ uint64_t myersPackFasta(const char *text, uint64_t textLen, uint32_t *out, uint6
  * (src/genReference-4bits.c:121-140).  Returns 0 or -1. */
 int myersWriteReferenceFile(const char *path, const uint32_t *words, uint32_t numEntries, uint32_t size);
 
+/* Read a .4bits.ref file: *words is malloc'ed (caller frees); returns 0 or -1. */
+int myersReadReferenceFile(const char *path, uint32_t **words, uint32_t *numEntries, uint32_t *size);
+
+/* A .gem.qry batch (src-api/genCandidate-reorder.c:353-390, read by
+ * src-api/test.c:30-66): u32 totalSizeQueries, totalQueriesEntries,
+ * sizeQueries, numQueries, numCandidates; qryEntry_t[E]; candInfo_t[C]
+ * (16 B); qryInfo_t[Q]; u32 gemScore[C]; char bases[totalSizeQueries]. */
+typedef struct {
+    uint32_t    totalSizeQueries;
+    uint32_t    numEntries;
+    uint32_t    numQueries;
+    uint32_t    numCandidates;
+    qryEntry_t *peq;
+    candInfo_t *cands;
+    qryInfo_t  *qinfo;
+    uint32_t   *gemScores;
+    char       *bases;
+} myersGemQry_t;
+
+int  myersReadGemQry(const char *path, myersGemQry_t *out);        /* 0 or -1 */
+int  myersWriteGemQry(const char *path, const myersGemQry_t *in);  /* 0 or -1 */
+void myersFreeGemQry(myersGemQry_t *q);
+
+/* Convert a GEM .prof file (one read per line: "<bases>\t<area>:<strand>:
+ * <position>:<score>\t..." as parsed by src-api/genCandidate-reorder.c:114-235)
+ * into a .gem.qry batch in memory: Peq entries per read (A/C/G/T in either
+ * case, every other letter N), candidates in file order. 0 or -1. */
+int  myersParseProf(const char *path, myersGemQry_t *out);
+
 #ifdef __cplusplus
 }
 #endif

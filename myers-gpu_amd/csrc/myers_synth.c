@@ -5,6 +5,7 @@
  * definition.  OpenMP; every random draw comes from counter-based streams,
  * so the output does not depend on the number of threads.
  */
+#define _GNU_SOURCE
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -363,4 +364,181 @@ fail:
     free(cand_off);
     free(code_off);
     return -1;
+}
+
+/* ------------------------------------------------------------- formats */
+
+int myersReadReferenceFile(const char *path, uint32_t **words, uint32_t *numEntries, uint32_t *size)
+{
+    FILE *fp = fopen(path, "rb");
+    if (fp == NULL) return -1;
+    uint32_t hdr[2];
+    int ok = fread(hdr, sizeof(uint32_t), 2, fp) == 2;
+    uint32_t *w = ok ? (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)hdr[0] + 1u)) : NULL;
+    ok = ok && w && fread(w, sizeof(uint32_t), hdr[0], fp) == hdr[0];
+    fclose(fp);
+    if (!ok) {
+        free(w);
+        return -1;
+    }
+    *words = w;
+    *numEntries = hdr[0];
+    *size = hdr[1];
+    return 0;
+}
+
+void myersFreeGemQry(myersGemQry_t *q)
+{
+    if (q == NULL) return;
+    free(q->peq);
+    free(q->cands);
+    free(q->qinfo);
+    free(q->gemScores);
+    free(q->bases);
+    memset(q, 0, sizeof(*q));
+}
+
+int myersReadGemQry(const char *path, myersGemQry_t *out)
+{
+    memset(out, 0, sizeof(*out));
+    FILE *fp = fopen(path, "rb");
+    if (fp == NULL) return -1;
+    uint32_t hdr[5];
+    int ok = fread(hdr, sizeof(uint32_t), 5, fp) == 5;
+    if (ok) {
+        out->totalSizeQueries = hdr[0];
+        out->numEntries = hdr[1];
+        out->numQueries = hdr[3];
+        out->numCandidates = hdr[4];
+        out->peq = (qryEntry_t *)malloc(sizeof(qryEntry_t) * ((size_t)hdr[1] + 1u));
+        out->cands = (candInfo_t *)malloc(sizeof(candInfo_t) * ((size_t)hdr[4] + 1u));
+        out->qinfo = (qryInfo_t *)malloc(sizeof(qryInfo_t) * ((size_t)hdr[3] + 1u));
+        out->gemScores = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)hdr[4] + 1u));
+        out->bases = (char *)malloc((size_t)hdr[0] + 1u);
+        ok = out->peq && out->cands && out->qinfo && out->gemScores && out->bases &&
+             fread(out->peq, sizeof(qryEntry_t), hdr[1], fp) == hdr[1] &&
+             fread(out->cands, sizeof(candInfo_t), hdr[4], fp) == hdr[4] &&
+             fread(out->qinfo, sizeof(qryInfo_t), hdr[3], fp) == hdr[3];
+        /* scores and bases are optional trailers (src-api/test.c stops after qinfo) */
+        if (ok) {
+            size_t got = fread(out->gemScores, sizeof(uint32_t), hdr[4], fp);
+            if (got != hdr[4]) memset(out->gemScores, 0, sizeof(uint32_t) * hdr[4]);
+            got = fread(out->bases, 1, hdr[0], fp);
+            if (got != hdr[0]) memset(out->bases, 0, hdr[0]);
+        }
+    }
+    fclose(fp);
+    if (!ok) {
+        myersFreeGemQry(out);
+        return -1;
+    }
+    return 0;
+}
+
+int myersWriteGemQry(const char *path, const myersGemQry_t *in)
+{
+    FILE *fp = fopen(path, "wb");
+    if (fp == NULL) return -1;
+    const uint32_t hdr[5] = {in->totalSizeQueries, in->numEntries, 0u, in->numQueries, in->numCandidates};
+    int ok = fwrite(hdr, sizeof(uint32_t), 5, fp) == 5 &&
+             fwrite(in->peq, sizeof(qryEntry_t), in->numEntries, fp) == in->numEntries &&
+             fwrite(in->cands, sizeof(candInfo_t), in->numCandidates, fp) == in->numCandidates &&
+             fwrite(in->qinfo, sizeof(qryInfo_t), in->numQueries, fp) == in->numQueries &&
+             (in->gemScores == NULL ||
+              fwrite(in->gemScores, sizeof(uint32_t), in->numCandidates, fp) == in->numCandidates) &&
+             (in->bases == NULL || fwrite(in->bases, 1, in->totalSizeQueries, fp) == in->totalSizeQueries);
+    return (fclose(fp) == 0 && ok) ? 0 : -1;
+}
+
+static int grow(void **p, size_t *cap, size_t need, size_t elem)
+{
+    if (need <= *cap) return 0;
+    size_t c = *cap ? *cap : 1024u;
+    while (c < need) c *= 2u;
+    void *n = realloc(*p, c * elem);
+    if (n == NULL) return -1;
+    *p = n;
+    *cap = c;
+    return 0;
+}
+
+int myersParseProf(const char *path, myersGemQry_t *out)
+{
+    memset(out, 0, sizeof(*out));
+    FILE *fp = fopen(path, "rb");
+    if (fp == NULL) return -1;
+    size_t cap_e = 0, cap_c = 0, cap_s = 0, cap_q = 0, cap_b = 0, ne = 0, nc = 0, nq = 0, nb = 0;
+    char *line = NULL;
+    size_t line_cap = 0;
+    ssize_t len;
+    int ok = 1;
+    while (ok && (len = getline(&line, &line_cap, fp)) > 0) {
+        while (len > 0 && (line[len - 1] == '\n' || line[len - 1] == '\r')) line[--len] = 0;
+        if (len == 0) continue;
+        /* the read is the first whitespace-free token, as sscanf("%s") takes it */
+        const size_t m = strcspn(line, " \t");
+        char *tab = line[m] ? line + m : NULL;
+        const uint32_t entries = (uint32_t)((m + 127u) / 128u);
+        if (grow((void **)&out->peq, &cap_e, ne + entries, sizeof(qryEntry_t)) ||
+            grow((void **)&out->qinfo, &cap_q, nq + 1u, sizeof(qryInfo_t)) ||
+            grow((void **)&out->bases, &cap_b, nb + m, 1u)) {
+            ok = 0;
+            break;
+        }
+        uint8_t *codes = (uint8_t *)malloc(m ? m : 1u);
+        if (codes == NULL) {
+            ok = 0;
+            break;
+        }
+        for (size_t i = 0; i < m; i++) {
+            codes[i] = (uint8_t)fasta_code((unsigned char)line[i]);
+            out->bases[nb + i] = line[i];
+        }
+        myersBuildPeq(codes, (uint32_t)m, out->peq + ne);
+        free(codes);
+        out->qinfo[nq].posEntry = (uint32_t)ne;
+        out->qinfo[nq].size = (uint32_t)m;
+        /* candidates: whitespace-separated "area:strand:position:score" */
+        char *p = tab;
+        while (p && *p) {
+            while (*p == '\t' || *p == ' ') p++;
+            if (!*p) break;
+            char *end = p;
+            while (*end && *end != '\t' && *end != ' ') end++;
+            char save = *end;
+            *end = 0;
+            char *c1 = strchr(p, ':');
+            char *c2 = c1 ? strchr(c1 + 1, ':') : NULL;
+            char *c3 = c2 ? strchr(c2 + 1, ':') : NULL;
+            if (c3 == NULL) {
+                ok = 0;
+                break;
+            }
+            if (grow((void **)&out->cands, &cap_c, nc + 1u, sizeof(candInfo_t)) ||
+                grow((void **)&out->gemScores, &cap_s, nc + 1u, sizeof(uint32_t))) {
+                ok = 0;
+                break;
+            }
+            out->cands[nc].query = (uint32_t)nq;
+            out->cands[nc].position = (uint64_t)strtoull(c2 + 1, NULL, 10);
+            out->gemScores[nc] = (uint32_t)strtoul(c3 + 1, NULL, 10);
+            nc++;
+            *end = save;
+            p = end;
+        }
+        ne += entries;
+        nq++;
+        nb += m;
+    }
+    free(line);
+    fclose(fp);
+    if (!ok) {
+        myersFreeGemQry(out);
+        return -1;
+    }
+    out->numEntries = (uint32_t)ne;
+    out->numQueries = (uint32_t)nq;
+    out->numCandidates = (uint32_t)nc;
+    out->totalSizeQueries = (uint32_t)nb;
+    return 0;
 }

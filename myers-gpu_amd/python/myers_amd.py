@@ -199,12 +199,26 @@ def tools():
         T.myersPackFasta.argtypes = [ctypes.c_char_p, u64, vp, u64]
         T.myersPackFasta.restype = u64
         T.myersWriteReferenceFile.argtypes = [ctypes.c_char_p, vp, u32, u32]
+        T.myersReadReferenceFile.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p),
+                                             ctypes.POINTER(u32), ctypes.POINTER(u32)]
+        T.myersReadGemQry.argtypes = [ctypes.c_char_p, ctypes.POINTER(GemQry)]
+        T.myersWriteGemQry.argtypes = [ctypes.c_char_p, ctypes.POINTER(GemQry)]
+        T.myersFreeGemQry.argtypes = [ctypes.POINTER(GemQry)]
+        T.myersFreeGemQry.restype = None
+        T.myersParseProf.argtypes = [ctypes.c_char_p, ctypes.POINTER(GemQry)]
         _tools = T
     return _tools
 
 
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
+
+
+class GemQry(ctypes.Structure):
+    _fields_ = [("totalSizeQueries", ctypes.c_uint32), ("numEntries", ctypes.c_uint32),
+                ("numQueries", ctypes.c_uint32), ("numCandidates", ctypes.c_uint32),
+                ("peq", ctypes.c_void_p), ("cands", ctypes.c_void_p), ("qinfo", ctypes.c_void_p),
+                ("gemScores", ctypes.c_void_p), ("bases", ctypes.c_void_p)]
 
 
 def check(rc, where):
@@ -235,6 +249,74 @@ def build_peq(codes):
     out = np.zeros(((len(codes) + 127) // 128, 20), dtype=np.uint32)
     tools().myersBuildPeq(_ptr(codes), len(codes), _ptr(out))
     return out
+
+
+def _copy_out(ptr, count, dtype, shape=None):
+    if count == 0 or not ptr:
+        return np.zeros(shape or (0,), dtype=dtype)
+    buf = (ctypes.c_char * (count * np.dtype(dtype).itemsize)).from_address(ptr)
+    a = np.frombuffer(buf, dtype=dtype).copy()
+    return a.reshape(shape) if shape else a
+
+
+def _gem_to_dict(g):
+    return {
+        "peq": _copy_out(g.peq, g.numEntries * 20, np.uint32, (g.numEntries, 20)),
+        "qinfo": _copy_out(g.qinfo, g.numQueries * 2, np.uint32, (g.numQueries, 2)),
+        "cands": _copy_out(g.cands, g.numCandidates, CAND_DTYPE),
+        "gem_scores": _copy_out(g.gemScores, g.numCandidates, np.uint32),
+        "bases": _copy_out(g.bases, g.totalSizeQueries, np.uint8).tobytes(),
+    }
+
+
+def read_reference_file(path):
+    """A .4bits.ref file -> (words, size)."""
+    w, ne, sz = ctypes.c_void_p(), ctypes.c_uint32(), ctypes.c_uint32()
+    if tools().myersReadReferenceFile(os.fsencode(path), ctypes.byref(w), ctypes.byref(ne), ctypes.byref(sz)) != 0:
+        raise OSError(f"cannot read reference file {path}")
+    try:
+        return _copy_out(w.value, ne.value, np.uint32), sz.value
+    finally:
+        ctypes.CDLL(None).free(w)
+
+
+def write_reference_file(path, words, size):
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    if tools().myersWriteReferenceFile(os.fsencode(path), _ptr(words), len(words), size) != 0:
+        raise OSError(f"cannot write reference file {path}")
+
+
+def parse_prof(path):
+    """A GEM .prof file -> dict(peq, qinfo, cands, gem_scores, bases)."""
+    g = GemQry()
+    if tools().myersParseProf(os.fsencode(path), ctypes.byref(g)) != 0:
+        raise ValueError(f"cannot parse .prof file {path}")
+    try:
+        return _gem_to_dict(g)
+    finally:
+        tools().myersFreeGemQry(ctypes.byref(g))
+
+
+def read_gem_qry(path):
+    g = GemQry()
+    if tools().myersReadGemQry(os.fsencode(path), ctypes.byref(g)) != 0:
+        raise ValueError(f"cannot read .gem.qry file {path}")
+    try:
+        return _gem_to_dict(g)
+    finally:
+        tools().myersFreeGemQry(ctypes.byref(g))
+
+
+def write_gem_qry(path, d):
+    peq = np.ascontiguousarray(d["peq"], dtype=np.uint32)
+    qinfo = np.ascontiguousarray(d["qinfo"], dtype=np.uint32)
+    cands = np.ascontiguousarray(d["cands"], dtype=CAND_DTYPE)
+    scores = np.ascontiguousarray(d.get("gem_scores", np.zeros(len(cands), np.uint32)), dtype=np.uint32)
+    bases = np.frombuffer(d.get("bases", b""), dtype=np.uint8).copy()
+    g = GemQry(len(bases), len(peq), len(qinfo), len(cands), peq.ctypes.data, cands.ctypes.data,
+               qinfo.ctypes.data, scores.ctypes.data, bases.ctypes.data if len(bases) else None)
+    if tools().myersWriteGemQry(os.fsencode(path), ctypes.byref(g)) != 0:
+        raise OSError(f"cannot write .gem.qry file {path}")
 
 
 def make_spec(seed, num_queries, *, length=None, choices=None, log_uniform=None, sub=0.0, ins=0.0,

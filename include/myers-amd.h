@@ -106,6 +106,19 @@ int myersSendEx(void *myersBuffer, float distance, uint32_t numPEQEntries, uint3
 /* receiveMyersBuffer returning an error code (src-api/myersGPU.c:515-524). */
 int myersReceiveEx(void *myersBuffer);
 
+/* Threshold mode (extension; semantics of src/checksum-2.c:88,104-105 and the
+ * cut-off search of GEMTools/src/gt_map_align_bpm.c:141-214).  With
+ * errorRate >= 0, every later batch on this buffer reports a candidate whose
+ * minimum score is <= (uint32)((float)m * errorRate) exactly as without the
+ * threshold, and every other verified candidate as
+ * {column 0xFFFFFFFF, score 0xFFFFFFFF}; skipped candidates keep
+ * {0, 0xFFFFFFFF}.  Kernels for queries of up to 1024 bases then skip the
+ * rows whose distance already exceeds the threshold (Ukkonen cut-off).
+ * errorRate < 0 switches the mode off (the default). */
+#define MYERS_NO_MATCH 0xFFFFFFFFu
+int   myersSetThreshold(void *myersBuffer, float errorRate);
+float myersGetThreshold(void *myersBuffer);
+
 /* endMyers returning an error code (src-api/myersGPU.c:651-676). */
 int myersEnd(void ***myersBuffer);
 

@@ -92,11 +92,12 @@ typedef struct myers_buffer {
     hipEvent_t   ev_fork, ev_join[SIDE_STREAMS];
     /* steady-state graph of the last device-resident batch shape */
     hipGraphExec_t graph_exec;
-    uint64_t     graph_key[10];
+    uint64_t     graph_key[11];
     int          graph_valid;
     int          graph_binned;
     uint32_t     graph_launches;
     int          exit_on_error;
+    float        threshold;       /* < 0: off (myersSetThreshold) */
     int          last_error;
     int          pending;         /* a batch was enqueued and not yet received */
     int          binned;
@@ -432,6 +433,7 @@ static int init_common(void ***out, uint32_t num_buffers, const uint32_t *words,
         b->max_queries = mq;
         b->max_peq_entries = mp;
         b->exit_on_error = cfg.exitOnError;
+        b->threshold = -1.0f;
         rc = init_buffer(b);
     }
     if (rc) {
@@ -576,6 +578,7 @@ typedef struct {
     void       *d_results;
     uint32_t    num_entries, num_queries, num_cands, flags;
     float       distance;
+    float       threshold;
 } batch_t;
 
 static uint32_t resolve_engine(uint32_t flags, const bin_scan_t *scan)
@@ -620,6 +623,7 @@ static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan
     a.cands = bt->d_cands;
     a.mode = bt->flags & 1u;
     a.distance = bt->distance;
+    a.threshold = bt->threshold;
     a.results = bt->d_results;
 
     const uint32_t engine = resolve_engine(bt->flags, scan);
@@ -712,14 +716,15 @@ static int enqueue_verify(myers_buffer_t *b, const batch_t *bt, const qryInfo_t 
         if ((rc = enqueue_pipeline(b, bt, &scan))) return rc;
         return timing_end(b, slot, b->stats.launches);
     }
-    uint32_t dist_bits;
+    uint32_t dist_bits, thr_bits;
     memcpy(&dist_bits, &bt->distance, sizeof(dist_bits));
-    const uint64_t key[10] = {(uint64_t)(uintptr_t)bt->d_peq, (uint64_t)(uintptr_t)bt->d_qinfo,
+    memcpy(&thr_bits, &bt->threshold, sizeof(thr_bits));
+    const uint64_t key[11] = {(uint64_t)(uintptr_t)bt->d_peq, (uint64_t)(uintptr_t)bt->d_qinfo,
                               (uint64_t)(uintptr_t)bt->d_cands, (uint64_t)(uintptr_t)bt->d_results,
                               ((uint64_t)bt->num_entries << 32) | bt->num_queries,
                               ((uint64_t)bt->num_cands << 32) | bt->flags,
                               (uint64_t)dist_bits, scan.present,
-                              (uint64_t)scan.too_long, b->ctx->tune_gen};
+                              (uint64_t)scan.too_long, b->ctx->tune_gen, (uint64_t)thr_bits};
     if (!b->graph_valid || memcmp(key, b->graph_key, sizeof(key)) != 0) {
         if (b->graph_exec) {
             hipGraphExecDestroy(b->graph_exec);
@@ -788,7 +793,7 @@ int myersSendEx(void *myersBuffer, float distance, uint32_t numPEQEntries, uint3
     if (numCandidates)
         SEND_TRY(hipMemcpyAsync(b->d_cands, b->h_cands, (size_t)numCandidates * sizeof(candInfo_t), hipMemcpyHostToDevice, s));
     const batch_t bt = {b->d_peq, b->d_qinfo, b->d_cands, b->d_results, numPEQEntries, numQueries,
-                        numCandidates, flags, distance};
+                        numCandidates, flags, distance, b->threshold};
     rc = enqueue_verify(b, &bt, b->h_qinfo);
     if (rc) return (b->last_error = rc);
     if (numCandidates)
@@ -869,13 +874,27 @@ int myersVerifyDevice(void *myersBuffer, const void *d_peq, uint32_t numPEQEntri
     if (rc) return (b->last_error = rc);
     *b->h_err = 0;
     const batch_t bt = {d_peq, d_qinfo, d_candidates, d_results, numPEQEntries, numQueries, numCandidates,
-                        flags, distance};
+                        flags, distance, b->threshold};
     rc = enqueue_verify(b, &bt, h_qinfo);
     b->stats.cells = 0;
     b->stats.candidates = numCandidates;
     b->pending = (rc == MYERS_SUCCESS);
     b->last_error = rc;
     return rc;
+}
+
+int myersSetThreshold(void *myersBuffer, float errorRate)
+{
+    myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
+    if (b == NULL || errorRate != errorRate) return MYERS_E_INVALID_ARGUMENT;
+    b->threshold = errorRate < 0.0f ? -1.0f : errorRate;
+    return MYERS_SUCCESS;
+}
+
+float myersGetThreshold(void *myersBuffer)
+{
+    const myers_buffer_t *b = (const myers_buffer_t *)myersBuffer;
+    return b == NULL ? -1.0f : b->threshold;
 }
 
 int myersSynchronize(void *myersBuffer)

@@ -49,6 +49,7 @@ struct CandRec {
 };
 
 constexpr uint32_t kSkipScore = 0xFFFFFFFFu;
+constexpr uint32_t kNoMatch = 0xFFFFFFFFu;  // threshold mode: {column, score} of a candidate above it
 constexpr uint32_t kZeroRow = 5u;          // LDS row of all-zero Peq words (L > 1)
 
 // n = (uint32)(m * (1 + 2*d)) in float32 with round-to-nearest steps
@@ -66,6 +67,11 @@ __device__ __forceinline__ uint32_t sanitize_text(uint32_t w) {
     uint32_t hi = (c >> 2) & 0x11111111u;
     uint32_t lo = (c | (c >> 1)) & 0x11111111u;
     return c & ~((hi & lo) * 7u);
+}
+
+// Under-threshold bound k = (uint32)((float)m * e) (src/checksum-2.c:88).
+__device__ __forceinline__ uint32_t threshold_of(uint32_t m, float e) {
+    return (uint32_t)__fmul_rn((float)m, e);
 }
 
 __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
@@ -194,6 +200,56 @@ __device__ __forceinline__ uint32_t from_lower_lane(uint32_t v, uint32_t not_fir
     return v;
 }
 
+// One column for L = 1.  Eq words stream from LDS one group of four words
+// ahead of use (group g+1 is read while group g computes; the caller reads
+// the next column's group 0 at the start of the column), so 12 Eq registers
+// are live instead of 2W.
+//
+// kCut: Ukkonen cut-off (Myers 1999, blocks model; GEMTools
+// gt_map_align_bpm.c:141-214): only the first ga groups are advanced.  ga is
+// the same in every lane, so a skipped group costs no issue slots; the group
+// test is made on a VGPR copy of ga (ga_v), which compiles to an exec mask
+// and a skip branch that keep every word in its register.  S is the lane's D
+// at the top row of group ga-1 (rows of the top-justified layout); without
+// kCut (all groups) S is the score row's D.  Either way the horizontal delta
+// leaving that row updates S.
+template <int W, bool kCut>
+__device__ __forceinline__ void column_l1(uint32_t (&Pv)[W], uint32_t (&Mv)[W], const uint4 &e0,
+                                          const uint4 *lds_lane, uint32_t cur_off, uint32_t ga_v, int32_t &S) {
+    using Sh = Shape<1, W>;
+    constexpr int kG = Sh::kGroups;
+    constexpr uint32_t kGS = (uint32_t)(Sh::kRows * Sh::kRowU4);     // uint4 between groups
+    uint32_t carry = 0, ph_prev = 0, mh_prev = 0;
+    uint4 cur = e0, nxt = e0;
+#pragma unroll
+    for (int g = 0; g < kG; g++) {
+        if (!kCut || g == 0 || (uint32_t)g < ga_v) {
+            if (g + 1 < kG) nxt = lds_lane[cur_off + (uint32_t)(g + 1) * kGS];
+            const uint32_t e4[4] = {cur.x, cur.y, cur.z, cur.w};
+#pragma unroll
+            for (int k = 4 * g; k < 4 * g + 4 && k < W; k++) {
+                const uint32_t pv = Pv[k], mv = Mv[k];
+                const uint32_t eq = e4[k - 4 * g];
+                const uint32_t xv = eq | mv;
+                uint32_t c_out;
+                const uint32_t sum = __builtin_addc(eq & pv, pv, carry, &c_out);
+                carry = c_out;
+                const uint32_t xh = __builtin_amdgcn_bitop3_b32(sum, pv, eq, kXorOr);
+                const uint32_t ph = __builtin_amdgcn_bitop3_b32(mv, xh, pv, kOrNor);
+                const uint32_t mh = pv & xh;
+                const uint32_t php = k == 0 ? (ph << 1) : __builtin_amdgcn_alignbit(ph, ph_prev, 31u);
+                const uint32_t mhp = k == 0 ? (mh << 1) : __builtin_amdgcn_alignbit(mh, mh_prev, 31u);
+                ph_prev = ph;
+                mh_prev = mh;
+                Pv[k] = __builtin_amdgcn_bitop3_b32(mhp, xv, php, kOrNor);
+                Mv[k] = php & xv;
+            }
+            cur = nxt;
+        }
+    }
+    S += (int32_t)(ph_prev >> 31) - (int32_t)(mh_prev >> 31);
+}
+
 template <int L, int W>
 struct Workspace {
     uint4 lds[Shape<L, W>::kLdsU4];
@@ -215,7 +271,8 @@ __device__ __forceinline__ void zero_row_init(Workspace<L, W> &ws) {
 
 // Verify chunks chunk_lo, chunk_lo + stride, ... < chunk_hi of the slot range
 // [start, start + count); a chunk is the kSlots candidates of one wave.
-template <int L, int W>
+// kCut (L = 1, W % 4 == 0): threshold mode with the Ukkonen cut-off.
+template <int L, int W, bool kCut>
 __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint32_t start, uint32_t count,
                                               uint32_t chunk_lo, uint32_t chunk_hi, uint32_t stride,
                                               Workspace<L, W> &ws) {
@@ -307,6 +364,18 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
             int32_t score = (int32_t)m, min_score = (int32_t)m;
             uint32_t min_col = 0;
             uint32_t ph_top = 0, mh_top = 0;
+            // Cut-off state: rows with D <= k (k + pad in layout rows, where
+            // the pad rows below the query hold D = row) must lie in the
+            // first ga groups, 8 rows below its top.  Start with every lane's
+            // rows 0..kv+8 covered; D at the top row R = 128 ga is then R.
+            constexpr uint32_t kG = (uint32_t)(W / 4);
+            int32_t kv = 0;
+            uint32_t ga = kG;
+            if constexpr (kCut) {
+                kv = (int32_t)threshold_of(m, a.threshold) + (32 * W - (int32_t)m);
+                ga = min(wave_max(ncols > 0u ? ((uint32_t)kv + 135u) / 128u : 1u), kG);
+                score = 128 * (int32_t)ga;
+            }
 
             // Text words stream through registers two blocks (16 columns) ahead
             // of their use; the Eq words of step t+1 are read from LDS while
@@ -333,31 +402,96 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
             uint32_t text = text_block(tw0, tw1, 0);
             uint32_t text_next = text_block(tw1, tw2, 1);
             uint32_t eq[W];
-            load_eq<L, W>(lds_lane, (text & 7u) * S::kRowU4, eq);
+            uint4 eq4 = make_uint4(0u, 0u, 0u, 0u);
+            uint32_t cur_off = (text & 7u) * S::kRowU4;
+            if constexpr (L == 1) {
+                eq4 = lds_lane[cur_off];
+            } else {
+                load_eq<L, W>(lds_lane, cur_off, eq);
+            }
             for (uint32_t b = 0; b < nblocks; b++) {
                 const uint32_t tw4 = text_word((int32_t)b + 4);
+                if constexpr (kCut) {
+                    // The extent changes only between blocks of 8 columns.  A row
+                    // with D <= k in column j + t lies at most t + 1 rows above
+                    // the last such row of column j - 1, so the extent must keep
+                    // 8 rows of headroom over it: rows R-7..R with D <= k force
+                    // D(R) <= k + 7 (one group is added), and a group can go
+                    // when its rows and the 8 below it exceed k: D(R) > k + 135.
+                    const bool on = 8u * b < ncols;
+                    if (ga < kG && __ballot(on && score <= kv + 7) != 0ull) {
+                        // new group: D rises by one per row above R (Pv = ~0, Mv = 0)
+#pragma unroll
+                        for (int g = 1; g < (int)kG; g++) {
+                            const bool sel = (uint32_t)g == ga;
+#pragma unroll
+                            for (int k = 4 * g; k < 4 * g + 4; k++) {
+                                Pv[k] = sel ? ~0u : Pv[k];
+                                Mv[k] = sel ? 0u : Mv[k];
+                            }
+                        }
+                        score += 128;
+                        ga++;
+                    } else if (ga > 1u && __ballot(on && score <= kv + 135) == 0ull) {
+                        // drop group ga-1: S moves down by its vertical deltas
+#pragma unroll
+                        for (int g = 1; g < (int)kG; g++) {
+                            if ((uint32_t)g + 1u == ga) {
+                                int32_t dv = 0;
+#pragma unroll
+                                for (int k = 4 * g; k < 4 * g + 4; k++)
+                                    dv += __builtin_popcount(Pv[k]) - __builtin_popcount(Mv[k]);
+                                score -= dv;
+                            }
+                        }
+                        ga--;
+                    }
+                }
+                uint32_t ga_v = ga;
+                if constexpr (kCut) asm volatile("v_mov_b32 %0, %1" : "=v"(ga_v) : "s"(ga));
 #pragma unroll
                 for (int i = 0; i < 8; i++) {
                     const uint32_t cn = (i < 7) ? ((text >> (4 * (i + 1))) & 7u) : (text_next & 7u);
-                    uint32_t eqn[W];
-                    load_eq<L, W>(lds_lane, cn * S::kRowU4, eqn);
+                    const uint32_t next_off = cn * S::kRowU4;
                     if constexpr (L == 1) {
-                        column_step<W, false>(Pv, Mv, eq, 0u, 0u, ph_top, mh_top);
+                        const uint4 eq4n = lds_lane[next_off];        // next column, group 0
+                        column_l1<W, kCut>(Pv, Mv, eq4, lds_lane, cur_off, ga_v, score);
+                        eq4 = eq4n;
+                        const uint32_t col = 8u * b + (uint32_t)i;
+                        const bool on = col < ncols;
+                        if constexpr (!kCut) {
+                            // first minimum of the score row (src-api/coreMyersGPU-CC35.cu:205-207,227-228)
+                            const bool better = (score < min_score) && on;
+                            min_col = better ? col : min_col;
+                            min_score = better ? score : min_score;
+                        } else {
+                            if (ga == kG) {
+                                // score is D at row m plus the pad rows below the query
+                                const int32_t d = score - (32 * W - (int32_t)m);
+                                const bool better = (d < min_score) && on;
+                                min_col = better ? col : min_col;
+                                min_score = better ? d : min_score;
+                            }
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
                     } else {
+                        uint32_t eqn[W];
+                        load_eq<L, W>(lds_lane, next_off, eqn);
                         const uint32_t ph_in = from_lower_lane<L>(ph_top, not_first);
                         const uint32_t mh_in = from_lower_lane<L>(mh_top, not_first);
                         column_step<W, true>(Pv, Mv, eq, ph_in, mh_in, ph_top, mh_top);
-                    }
-                    // score row and first minimum (src-api/coreMyersGPU-CC35.cu:205-207,227-228);
-                    // only the group's top lane keeps them, at column t - (L-1)
-                    score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
-                    const uint32_t col = 8u * b + (uint32_t)i - (uint32_t)(L - 1);
-                    const bool better = (score < min_score) && (col < ncols);
-                    min_col = better ? col : min_col;
-                    min_score = better ? score : min_score;
-                    __builtin_amdgcn_sched_barrier(0);
+                        // score row and first minimum (src-api/coreMyersGPU-CC35.cu:205-207,227-228);
+                        // only the group's top lane keeps them, at column t - (L-1)
+                        score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
+                        const uint32_t col = 8u * b + (uint32_t)i - (uint32_t)(L - 1);
+                        const bool better = (score < min_score) && (col < ncols);
+                        min_col = better ? col : min_col;
+                        min_score = better ? score : min_score;
+                        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int k = 0; k < W; k++) eq[k] = eqn[k];
+                        for (int k = 0; k < W; k++) eq[k] = eqn[k];
+                    }
+                    cur_off = next_off;
                 }
                 text = text_next;
                 text_next = text_block(tw2, tw3, b + 2);
@@ -366,14 +500,18 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
             }
 
             if (in_pass && valid && gl == (uint32_t)(L - 1)) {
-                results[cand] = live ? make_uint2(min_col, (uint32_t)min_score) : make_uint2(0u, kSkipScore);
+                uint2 r = live ? make_uint2(min_col, (uint32_t)min_score) : make_uint2(0u, kSkipScore);
+                // threshold mode: score <= (uint32)((float)m * e) (src/checksum-2.c:88,104-105)
+                if (live && a.threshold >= 0.0f && (uint32_t)min_score > threshold_of(m, a.threshold))
+                    r = make_uint2(kNoMatch, kNoMatch);
+                results[cand] = r;
             }
         }
     }
 }
 
 // Plain launch: a fixed grid strides over the chunks of one slot range.
-template <int L, int W>
+template <int L, int W, bool kCut>
 __global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
     using S = Shape<L, W>;
     __shared__ Workspace<L, W> ws;
@@ -389,13 +527,13 @@ __global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
     // candidates of one query -- and their Peq -- in one L2.
     uint32_t first = blockIdx.x;
     if ((gridDim.x & 7u) == 0u) first = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
-    verify_chunks<L, W>(a, start, count, first, nchunks, gridDim.x, ws);
+    verify_chunks<L, W, kCut>(a, start, count, first, nchunks, gridDim.x, ws);
 }
 
 // Persistent launch: a resident grid pulls work items of `chunk_factor`
 // chunks from the variant's device queue (an atomic head reset by the
 // binning) until the bin is drained; every wave reaches the exit test.
-template <int L, int W>
+template <int L, int W, bool kCut>
 __global__ __launch_bounds__(64) void verify_persistent_kernel(myers_verify_args_t a) {
     using S = Shape<L, W>;
     __shared__ Workspace<L, W> ws;
@@ -411,7 +549,7 @@ __global__ __launch_bounds__(64) void verify_persistent_kernel(myers_verify_args
         __syncthreads();
         const uint32_t lo = __builtin_amdgcn_readfirstlane(item_sh);
         if (lo >= nchunks) break;
-        verify_chunks<L, W>(a, start, count, lo, min(lo + cf, nchunks), 1u, ws);
+        verify_chunks<L, W, kCut>(a, start, count, lo, min(lo + cf, nchunks), 1u, ws);
     }
 }
 
@@ -520,10 +658,24 @@ struct VariantEntry {
     uint32_t lanes, words;
     const void *fn;
     const void *persistent_fn;
+    const void *cut_fn;              // threshold mode with the cut-off, or null
+    const void *cut_persistent_fn;
 };
+template <int L, int W>
+constexpr bool kHasCut = L == 1 && W >= 8 && W % 4 == 0;
+template <int L, int W>
+constexpr const void *cut_kernel(bool persistent) {
+    if constexpr (kHasCut<L, W>) {
+        return persistent ? reinterpret_cast<const void *>(&verify_persistent_kernel<L, W, true>)
+                          : reinterpret_cast<const void *>(&verify_kernel<L, W, true>);
+    } else {
+        return nullptr;
+    }
+}
 #define MYERS_VARIANT_ENTRY(L, W)                                                                \
-    {L, W, reinterpret_cast<const void *>(&verify_kernel<L, W>),                                \
-     reinterpret_cast<const void *>(&verify_persistent_kernel<L, W>)},
+    {L, W, reinterpret_cast<const void *>(&verify_kernel<L, W, false>),                         \
+     reinterpret_cast<const void *>(&verify_persistent_kernel<L, W, false>), cut_kernel<L, W>(false), \
+     cut_kernel<L, W>(true)},
 const VariantEntry kVariants[] = {MYERS_VARIANT_LIST(MYERS_VARIANT_ENTRY)};
 constexpr uint32_t kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 static_assert(kNumVariants <= MYERS_MAX_VARIANTS, "variant table too large");
@@ -566,8 +718,10 @@ extern "C" int myers_launch_verify(uint32_t v, const myers_verify_args_t *args, 
     if (v >= kNumVariants || args == nullptr || grid == 0) return -1;
     if (persistent && (args->bin_ranges == nullptr || args->queue_heads == nullptr)) return -1;
     void *kargs[] = {const_cast<myers_verify_args_t *>(args)};
-    hipError_t e = hipLaunchKernel(persistent ? kVariants[v].persistent_fn : kVariants[v].fn, dim3(grid),
-                                   dim3(64), kargs, 0, stream);
+    const VariantEntry &ve = kVariants[v];
+    const bool cut = args->threshold >= 0.0f && ve.cut_fn != nullptr;
+    const void *fn = cut ? (persistent ? ve.cut_persistent_fn : ve.cut_fn) : (persistent ? ve.persistent_fn : ve.fn);
+    hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(64), kargs, 0, stream);
     return (int)e;
 }
 

@@ -165,6 +165,9 @@ def lib():
         L.myersGetBinTable.argtypes = [vp, vp, u32]
         L.myersSetBinTable.argtypes = [vp, vp, u32]
         L.myersSetChunkFactor.argtypes = [vp, u32, u32]
+        L.myersSetThreshold.argtypes = [vp, ctypes.c_float]
+        L.myersGetThreshold.argtypes = [vp]
+        L.myersGetThreshold.restype = ctypes.c_float
         L.myersGetChunkFactor.argtypes = [vp, u32, ctypes.POINTER(ctypes.c_uint32)]
         L.myersLoadTuning.argtypes = [vp, ctypes.c_char_p]
         L.myersServeDefaultConfig.argtypes = [ctypes.POINTER(ServeConfig)]
@@ -464,6 +467,14 @@ class Myers:
         t = np.zeros(max_words + 1, dtype=np.uint8)
         check(lib().myersGetBinTable(self.buffer(0), _ptr(t), max_words), "myersGetBinTable")
         return t
+
+    def set_threshold(self, error_rate, i=None):
+        """Threshold mode on buffer i (all buffers if None); error_rate < 0 switches it off."""
+        for j in (range(self.num_buffers) if i is None else [i]):
+            check(lib().myersSetThreshold(self.buffer(j), error_rate), "myersSetThreshold")
+
+    def threshold(self, i=0):
+        return lib().myersGetThreshold(self.buffer(i))
 
     def set_chunk_factor(self, variant, chunks):
         check(lib().myersSetChunkFactor(self.buffer(0), variant, chunks), "myersSetChunkFactor")

@@ -22,6 +22,28 @@ uint32_t oracle_window_len(uint32_t m, float distance)
     return (uint32_t)prod;
 }
 
+uint32_t oracle_threshold(uint32_t m, float e)
+{
+    /* src/checksum-2.c:88: `uint threshold = distance * querySize;` with a
+     * float distance: the uint converts to float, the product is a float32
+     * rounding, the assignment truncates. */
+    volatile float prod = (float)m * e;
+    return (uint32_t)prod;
+}
+
+void oracle_apply_threshold(const oracle_qinfo_t *qinfo, const oracle_cand_t *cands, uint64_t n, float e,
+                            oracle_result_t *results)
+{
+    for (uint64_t i = 0; i < n; i++) {
+        if (results[i].score == 0xFFFFFFFFu) continue;          /* skipped */
+        const uint32_t m = qinfo[cands[i].query].size;
+        if (results[i].score > oracle_threshold(m, e)) {
+            results[i].column = 0xFFFFFFFFu;
+            results[i].score = 0xFFFFFFFFu;
+        }
+    }
+}
+
 /* Nibble idx of the packed text (src/genReference-4bits.c:43-56 layout). */
 static inline uint32_t text_code(const uint32_t *ref, uint64_t nw, uint64_t idx)
 {

@@ -20,6 +20,8 @@
  *                         all-mismatch) and the KAT
  *   oracle_pack_reference golden .4bits.ref written by src/genReference-4bits.c
  *   oracle_build_peq      golden .gem.qry written by src-api/genCandidate-reorder.c
+ *   oracle_threshold      thresholds read back from src/checksum-2.c's own
+ *                         counts (tests/golden/threshold_kat.json)
  */
 #ifndef MYERS_ORACLE_H
 #define MYERS_ORACLE_H
@@ -37,6 +39,17 @@ typedef struct { uint32_t pos_entry; uint32_t size; } oracle_qinfo_t;       /* 8
 typedef struct { uint32_t column; uint32_t score; } oracle_result_t;        /* 8 B  */
 
 enum { ORACLE_MODE_ALIGNED = 0, ORACLE_MODE_EXACT = 1 };
+
+/* Under-threshold bound (uint32)((float)m * e), the float product of
+ * src/checksum-2.c:88 (float distance times uint query size). */
+uint32_t oracle_threshold(uint32_t m, float e);
+
+/* Threshold mode: a verified candidate whose score exceeds
+ * oracle_threshold(m, e) becomes {0xFFFFFFFF, 0xFFFFFFFF}; under-threshold
+ * and skipped ({0, 0xFFFFFFFF}) candidates are left as they are
+ * (src/checksum-2.c:104-105 counts score <= threshold as a match). */
+void oracle_apply_threshold(const oracle_qinfo_t *qinfo, const oracle_cand_t *cands, uint64_t n, float e,
+                            oracle_result_t *results);
 
 /* Window length n = (uint32)(m * (1 + 2*d)) evaluated in float32
  * (src-api/coreMyersGPU-CC35.cu:131, src/myers-4bits.c:163). */

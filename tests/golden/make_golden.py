@@ -14,8 +14,11 @@ Programs and what their outputs pin:
                         aligned windows, variable lengths, 128-base Peq entries
   myers4bitsCPU         (src/myers-4bits.c)            exact-start windows, fixed length
   myers4bitsCPU-padding (src/myers-4bits-padding.c)    aligned windows, fixed length
+  checksum-2            (src/checksum-2.c)              under-threshold counts, read back
+                        as the threshold (uint32)((float)m * e) per (e, m)
 
-Usage: python tests/golden/make_golden.py   (writes tests/golden/*.npz)
+Usage: python tests/golden/make_golden.py [threshold]
+       (writes tests/golden/*.npz and threshold_kat.json; "threshold" only the latter)
 """
 import os
 import shutil
@@ -259,12 +262,49 @@ def make_fixed_length(tmp, rng, name, size, m, nq, distance, program, suffix, mo
     return nc
 
 
+def make_threshold_kat(tmp):
+    """For each (e, m), results with scores 0..top go through checksum-2 twice
+    (old = new); its count of scores <= threshold is threshold + 1."""
+    import json
+    import re
+    pairs = [(e, m) for e in ("0", "0.02", "0.05", "0.1", "0.15", "0.2", "0.3", "0.7", "1", "1.5")
+             for m in (1, 7, 10, 30, 64, 100, 150, 333, 1000, 1023, 4096, 20000)]
+    # pairs where a float32 product, a double product of the decimal rate and a
+    # double product of the float32 rate disagree
+    pairs += [("0.7", 90), ("0.7", 170), ("0.12", 25), ("0.08", 25), ("0.35", 20), ("0.45", 20),
+              ("0.9", 10), ("0.9", 20)]
+    out = []
+    for e, m in pairs:
+        path = os.path.join(tmp, f"res_{e}_{m}.txt")
+        top = int(m * max(float(e), 1.0)) + 2
+        with open(path, "w") as f:
+            f.write(f"{top + 1}\n")
+            for i in range(top + 1):
+                f.write(f"{i} {i} 0\n")
+        r = subprocess.run([os.path.join(REF_BIN, "checksum-2"), e, str(m), path, path], check=True,
+                           capture_output=True, text=True)
+        shown = int(re.search(r"with threshold (\d+)", r.stdout).group(1))
+        under = int(re.search(r"New Matching Candidates: (\d+)", r.stdout).group(1))
+        assert under == shown + 1, (e, m, r.stdout)
+        out.append({"e": e, "m": m, "threshold": shown})
+    with open(os.path.join(HERE, "threshold_kat.json"), "w") as f:
+        json.dump(out, f, indent=0)
+    return len(out)
+
+
 def main():
     if not os.path.isdir("/root/reference"):
         sys.exit("the reference sources are not present here; fixtures are already committed")
     build_ref_programs()
     tmp = tempfile.mkdtemp(prefix="golden_")
+    if sys.argv[1:] == ["threshold"]:
+        try:
+            print("threshold pairs written:", make_threshold_kat(tmp))
+        finally:
+            shutil.rmtree(tmp)
+        return
     try:
+        make_threshold_kat(tmp)
         rng = np.random.default_rng(20140501)
         edge = [1, 2, 7, 8, 31, 32, 33, 63, 64, 65, 96, 100, 127, 128, 129, 150, 200, 255,
                 256, 257, 300, 511, 512, 513, 1000, 1023, 1024, 1025, 2000, 4095, 4096, 4097, 5000]

@@ -42,6 +42,10 @@ def load():
     lib.oracle_dp_trace.argtypes = [vp, u32, vp, u32, vp, vp]
     lib.oracle_pack_reference.argtypes = [vp, u64, vp]
     lib.oracle_build_peq.argtypes = [vp, u32, vp]
+    lib.oracle_threshold.argtypes = [u32, f32]
+    lib.oracle_threshold.restype = u32
+    lib.oracle_apply_threshold.argtypes = [vp, vp, u64, f32, vp]
+    lib.oracle_apply_threshold.restype = None
     _lib = lib
     return lib
 
@@ -66,6 +70,19 @@ def myers_batch(ref_words, ref_size, peq, qinfo, cands, distance, mode=MODE_ALIG
     out = np.zeros((len(cands), 2), dtype=np.uint32)
     load().oracle_myers_batch(_ptr(ref_words), len(ref_words), int(ref_size), _ptr(peq), _ptr(qinfo),
                               _ptr(cands), len(cands), float(distance), mode, max_query_len, _ptr(out))
+    return out
+
+
+def threshold(m, e):
+    return int(load().oracle_threshold(int(m), float(e)))
+
+
+def apply_threshold(qinfo, cands, results, e):
+    """Copy of results with the threshold mode's no-match sentinel applied."""
+    qinfo = np.ascontiguousarray(qinfo, dtype=np.uint32)
+    cands = np.ascontiguousarray(cands, dtype=CAND_DTYPE)
+    out = np.ascontiguousarray(results, dtype=np.uint32).copy()
+    load().oracle_apply_threshold(_ptr(qinfo), _ptr(cands), len(cands), float(e), _ptr(out))
     return out
 
 

@@ -1,0 +1,122 @@
+"""Threshold mode (myersSetThreshold): bit-exact results for candidates whose
+score is <= (uint32)((float)m * e), {0xFFFFFFFF, 0xFFFFFFFF} for the other
+verified candidates, the skip sentinel unchanged.  The oracle computes every
+candidate in full and applies the threshold afterwards; the kernels for
+queries of up to 1024 bases skip rows above the threshold (Ukkonen cut-off),
+so equality here checks that the cut-off loses nothing under the threshold.
+"""
+import numpy as np
+import pytest
+
+import myers_amd as M
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+RATES = (0.0, 0.05, 0.1, 0.2, 0.35, 1.0)
+
+
+def announce(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def expected(ref, size, b, distance, mode, e):
+    full = O.myers_batch(ref, size, b["peq"], b["qinfo"], b["cands"], distance, mode)
+    return O.apply_threshold(b["qinfo"], b["cands"], full, e)
+
+
+def test_threshold_every_single_lane_variant():
+    """Every one-lane variant (cut-off kernels for W = 8..32 in steps of 4,
+    post-filtered full kernels for the others), lengths across the variant's
+    range in one wave, error rates around the thresholds."""
+    size = 2_000_000
+    ref = M.synth_reference(61, size)
+    with M.Myers(ref, size, avg_query=600, cands_per_query=6, buffer_bytes=256 << 20) as m:
+        variants = m.variants()
+        default = m.bin_table()
+        for v, info in enumerate(variants):
+            if info["lanes"] != 1:
+                continue
+            cap = info["maxQuerySize"]
+            lo = max(1, cap - 32 * 4 + 1)
+            spec = M.make_spec(300 + v, 40, choices=sorted({lo, (lo + cap) // 2, cap - 3, cap}),
+                               err_range=(0.0, 0.4), sub_frac=0.6, candidates=6, distance=0.1)
+            b = M.synth_batch(spec, ref, size)
+            table = default.copy()
+            table[:] = v
+            table[(np.arange(len(table)) * 32) > cap] = default[(np.arange(len(table)) * 32) > cap]
+            m.set_bin_table(table)
+            for e in RATES:
+                announce(f"threshold variant W={info['words']} e={e}")
+                m.set_threshold(e)
+                got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1)
+                np.testing.assert_array_equal(got, expected(ref, size, b, 0.1, 0, e),
+                                              err_msg=f"variant {info} e={e}")
+        m.set_threshold(-1.0)
+        m.set_bin_table(default)
+
+
+@pytest.mark.parametrize("mode", [M.WINDOW_ALIGNED, M.WINDOW_EXACT])
+def test_threshold_config2_shape(mode):
+    """1000-base reads, 11 candidates each (one true locus), e = 0.2, both engines."""
+    size = 3_000_000
+    ref = M.synth_reference(62, size)
+    b = M.synth_batch(M.make_spec(63, 3000, length=1000, sub=0.03, ins=0.005, dele=0.005, candidates=11,
+                                  distance=0.1), ref, size)
+    want = expected(ref, size, b, 0.1, mode, 0.2)
+    under = np.mean(want[:, 1].reshape(-1, 11)[:, 0] != 0xFFFFFFFF)
+    assert under > 0.99                       # the true loci stay under the threshold
+    with M.Myers(ref, size, avg_query=1000, cands_per_query=11, buffer_bytes=256 << 20) as m:
+        m.set_threshold(0.2)
+        assert abs(m.threshold() - 0.2) < 1e-7
+        for engine in (M.ENGINE_PLAIN, M.ENGINE_PERSISTENT):
+            announce(f"threshold config-2 shape mode={mode} engine={engine}")
+            got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=mode | engine)
+            np.testing.assert_array_equal(got, want)
+        m.set_threshold(-1.0)
+        got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=mode)
+        np.testing.assert_array_equal(got, O.myers_batch(ref, size, b["peq"], b["qinfo"], b["cands"], 0.1, mode))
+
+
+def test_threshold_mixed_lengths_and_long_reads():
+    """Mixed short reads and long reads (multi-lane kernels post-filter) in
+    one batch, with skip-rule candidates at the reference end."""
+    size = 3_000_000
+    ref = M.synth_reference(64, size)
+    b = M.synth_batch(M.make_spec(65, 900, choices=[50, 100, 200, 333, 600, 1000, 1500, 3000],
+                                  err_range=(0.01, 0.3), sub_frac=0.7, poisson=9, cap=40, distance=0.1),
+                      ref, size)
+    cands = b["cands"].copy()
+    cands["position"][::97] = size - 5                    # skipped: window past the end
+    b = dict(b, cands=cands)
+    with M.Myers(ref, size, avg_query=800, cands_per_query=9, buffer_bytes=256 << 20) as m:
+        for e in (0.02, 0.1, 0.25):
+            announce(f"threshold mixed e={e}")
+            m.set_threshold(e)
+            got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1)
+            want = expected(ref, size, b, 0.1, 0, e)
+            np.testing.assert_array_equal(got, want)
+            assert np.all(got[::97] == [0, 0xFFFFFFFF])
+
+
+def test_threshold_graph_recaptures_on_change():
+    """A captured pipeline is keyed by the threshold: changing it captures anew."""
+    announce("threshold graph")
+    size = 2_000_000
+    ref = M.synth_reference(66, size)
+    b = M.synth_batch(M.make_spec(67, 600, choices=[100, 400, 1000], err_range=(0.01, 0.2), poisson=11,
+                                  distance=0.1), ref, size)
+    with M.Myers(ref, size, avg_query=500, cands_per_query=11, buffer_bytes=256 << 20) as m:
+        dp, dq, dc = M.DeviceArray(b["peq"]), M.DeviceArray(b["qinfo"]), M.DeviceArray(b["cands"])
+        dr = M.DeviceArray.empty((len(b["cands"]), 2), np.uint32)
+        nq, nc, ne = len(b["qinfo"]), len(b["cands"]), len(b["peq"])
+        for i, e in enumerate((0.1, 0.1, 0.05, -1.0)):
+            m.set_threshold(e)
+            M.verify_device(m, dp, ne, dq, nq, dc, nc, 0.1, dr, flags=M.USE_GRAPH, h_qinfo=b["qinfo"])
+            M.synchronize(m)
+            full = O.myers_batch(ref, size, b["peq"], b["qinfo"], b["cands"], 0.1, 0)
+            want = full if e < 0 else O.apply_threshold(b["qinfo"], b["cands"], full, e)
+            np.testing.assert_array_equal(dr.get(), want, err_msg=f"call {i} e={e}")
+        assert m.stats().graphCaptures == 3
+        for x in (dp, dq, dc, dr):
+            x.free()

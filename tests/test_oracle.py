@@ -185,3 +185,27 @@ def test_myers_trace_equals_dp_random():
         a = O.myers_trace(q, t)
         b = O.dp_trace(q, t)
         assert a[0].tolist() == b[0].tolist() and a[1] == b[1]
+
+
+def test_threshold_matches_reference_checksum():
+    """(uint32)((float)m * e) as src/checksum-2.c computes it, read back from
+    the reference program's own under-threshold counts; the set holds pairs
+    where float32, double and decimal products disagree."""
+    kat = json.load(open(os.path.join(GOLDEN, "threshold_kat.json")))
+    assert len(kat) >= 120
+    for x in kat:
+        assert O.threshold(x["m"], float(x["e"])) == x["threshold"], x
+    # a double product of the float32 rate would give 6 here, the reference gives 7
+    assert {(x["e"], x["m"]): x["threshold"] for x in kat}[("0.7", 10)] == 7
+
+
+def test_apply_threshold_sentinels():
+    qinfo = np.array([[0, 100], [1, 10]], dtype=np.uint32)
+    cands = np.zeros(5, O.CAND_DTYPE)
+    cands["query"] = [0, 0, 0, 1, 1]
+    res = np.array([[5, 20], [6, 21], [0, 0xFFFFFFFF], [3, 7], [4, 8]], dtype=np.uint32)
+    out = O.apply_threshold(qinfo, cands, res, 0.2)
+    np.testing.assert_array_equal(out, [[5, 20], [0xFFFFFFFF, 0xFFFFFFFF], [0, 0xFFFFFFFF],
+                                        [0xFFFFFFFF, 0xFFFFFFFF], [0xFFFFFFFF, 0xFFFFFFFF]])
+    out = O.apply_threshold(qinfo, cands, res, 0.7)
+    np.testing.assert_array_equal(out[3:], [[3, 7], [0xFFFFFFFF, 0xFFFFFFFF]])

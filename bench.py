@@ -74,6 +74,11 @@ def algorithmic_bytes(qinfo, cands):
     return int(per_cand.sum() + per_read.sum())
 
 
+def threshold_of(m, e):
+    """(uint32)((float)m * e), the float32 product of src/checksum-2.c:88."""
+    return int(np.float32(m) * np.float32(e))
+
+
 def cpu_baseline(ref, ref_size, b, gpu_results, seconds):
     """Time the oracle (C restatement of src/myers-4bits*.c, OpenMP) on a
     seeded sample of the same candidates and check the GPU results on it."""
@@ -100,6 +105,12 @@ def cpu_baseline(ref, ref_size, b, gpu_results, seconds):
     m = b["qinfo"][cands["query"][idx], 1].astype(np.int64)
     cells = int((m * window_len(m)).sum())
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    # GEMTools' block BPM over 64-bit words on the same sample (a second CPU baseline)
+    t0 = time.perf_counter()
+    res64 = O.bpm64_batch(ref, ref_size, b["peq"], b["qinfo"], cands[idx], DISTANCE, O.MODE_ALIGNED)
+    elapsed64 = time.perf_counter() - t0
+    if not np.array_equal(res64, res):
+        raise SystemExit("bench: the 64-bit BPM baseline differs from the oracle")
     return {
         "value": checked / elapsed,
         "unit": "candidates/s",
@@ -109,6 +120,10 @@ def cpu_baseline(ref, ref_size, b, gpu_results, seconds):
         "sample": f"{checked} seeded-random candidates of the same batch ({elapsed:.1f} s, OpenMP static); "
                   f"GPU results bit-exact on all of them",
         "cpu_model": _cpu_model(),
+        "algorithm": "u32 blocks with int8 carry (src/myers-4bits.c)",
+        "bpm64": {"value": checked / elapsed64, "unit": "candidates/s", "gcups": cells / elapsed64 / 1e9,
+                  "cores": threads, "kind": "port",
+                  "algorithm": "64-bit block BPM (GEMTools/src/gt_map_align_bpm_generic.c)"},
     }
 
 
@@ -246,6 +261,8 @@ def main():
     ap.add_argument("--serve-loads", default="0.7", help="serving load levels (fractions of saturation); '' = skip")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--variant", default=None, help="force a kernel variant for every read length, e.g. L2xW16")
+    ap.add_argument("--threshold", type=float, default=0.2,
+                    help="also time threshold mode at this error rate (negative = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (candidates would shard by read "
@@ -312,6 +329,26 @@ def main():
     used = sorted({int(v) for v in m.bin_table()[np.unique((b["qinfo"][:, 1] + 31) // 32)]})
     variants = [f"L{info[v]['lanes']}xW{info[v]['words']}" for v in used]
 
+    threshold = None
+    if args.threshold >= 0:
+        m.set_threshold(args.threshold)
+        r = run_steps(m, bufs, b, flags, max(3, args.steps // 2), 1)
+        m.set_threshold(-1.0)
+        tres = bufs[3].get()
+        # same results as the full run for candidates under the threshold, the no-match pair otherwise
+        t = np.array([threshold_of(x, args.threshold) for x in range(int(b["qinfo"][:, 1].max()) + 1)],
+                     dtype=np.int64)[b["qinfo"][b["cands"]["query"], 1]]
+        under = (results[:, 1] != 0xFFFFFFFF) & (results[:, 1].astype(np.int64) <= t)
+        skipped = results[:, 1] == 0xFFFFFFFF
+        ok = (np.array_equal(tres[under], results[under]) and np.array_equal(tres[skipped], results[skipped])
+              and np.all(tres[~under & ~skipped] == 0xFFFFFFFF))
+        if not ok:
+            raise SystemExit("bench: threshold-mode results differ from the full run")
+        threshold = {"error_rate": args.threshold, "candidates_per_s": r["cand_per_s"],
+                     "kernel_ms": r["kernel_ms"], "under_threshold": int(under.sum()),
+                     "speedup_vs_full": r["cand_per_s"] / (nc * args.steps / run["elapsed"])}
+        log(f"threshold mode: {threshold}")
+
     cpu = None
     if args.cpu_seconds > 0:
         log("CPU baseline / parity sample ...")
@@ -359,6 +396,7 @@ def main():
                     "frac": hbm_achieved / PEAK_HBM_GBS, "algorithmic_bytes": hbm_bytes},
         },
         "engine_comparison": comparison,
+        "threshold_mode": threshold,
         "cpu_baseline": cpu,
         "serving": serving,
     }

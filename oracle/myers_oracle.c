@@ -130,6 +130,112 @@ static void verify_candidate(const uint32_t *ref, uint64_t nw, uint64_t size,
     res->score = (uint32_t)min_score;
 }
 
+/*
+ * One candidate with GEMTools' block BPM over 64-bit words
+ * (GEMTools/src/gt_map_align_bpm_generic.c:26-121): per block the
+ * horizontal delta leaving the block's last query row (its level mask) is
+ * the next block's hin, injected as Eq |= MHin, Ph |= PHin, Mh |= MHin, and
+ * each block keeps the score of its last row.  Window and skip rules as
+ * verify_candidate.  A second, independent formulation of the same DP, used
+ * as a CPU baseline next to the reference's u32 code.
+ */
+static void verify_candidate_bpm64(const uint32_t *ref, uint64_t nw, uint64_t size,
+                                   const oracle_peq_entry_t *qpeq, uint32_t m, uint64_t pos, uint32_t n,
+                                   int mode, uint64_t *work, oracle_result_t *res)
+{
+    res->column = 0;
+    res->score = 0xFFFFFFFFu;
+    if (!((pos < size) && ((size - pos) > (uint64_t)n)))
+        return;
+    if (m == 0u) {                             /* empty query: D[0][j] = 0 */
+        res->score = 0u;
+        return;
+    }
+    const uint32_t blocks = (m + 63u) / 64u;
+    uint64_t *peq = work;                      /* [5][blocks] */
+    uint64_t *P = peq + 5u * blocks, *M = P + blocks, *mask = M + blocks;
+    int64_t *score = (int64_t *)(mask + blocks);
+    for (uint32_t c = 0; c < 5u; c++)
+        for (uint32_t b = 0; b < blocks; b++) {
+            const uint32_t w0 = 2u * b, w1 = 2u * b + 1u;
+            const uint64_t lo = qpeq[w0 / 4u].bitmap[c][w0 % 4u];
+            const uint64_t hi = (w1 < (m + 31u) / 32u) ? qpeq[w1 / 4u].bitmap[c][w1 % 4u] : 0u;
+            peq[c * blocks + b] = lo | (hi << 32);
+        }
+    for (uint32_t b = 0; b < blocks; b++) {
+        const uint32_t rows = (b + 1u == blocks) ? m - 64u * b : 64u;
+        mask[b] = 1ull << (rows - 1u);
+        P[b] = ~0ull;
+        M[b] = 0u;
+        score[b] = (b ? score[b - 1] : 0) + (int64_t)rows;
+    }
+    uint64_t start;
+    uint32_t ncols;
+    if (mode == ORACLE_MODE_ALIGNED) {
+        start = (pos / 8u) * 8u;
+        ncols = 8u * ((n / 8u) + ((n % 8u) ? 2u : 1u));
+    } else {
+        start = pos;
+        ncols = n;
+    }
+    int64_t min_score = INT64_MAX;
+    uint32_t min_col = 0;
+    for (uint32_t j = 0; j < ncols; j++) {
+        const uint32_t c = peq_row(text_code(ref, nw, start + j));
+        uint64_t phin = 0, mhin = 0;
+        for (uint32_t b = 0; b < blocks; b++) {
+            const uint64_t pv = P[b], mv = M[b], eq = peq[c * blocks + b];
+            const uint64_t xv = eq | mv;
+            const uint64_t eqh = eq | mhin;
+            const uint64_t xh = (((eqh & pv) + pv) ^ pv) | eqh;
+            uint64_t ph = mv | ~(xh | pv);
+            uint64_t mh = pv & xh;
+            const uint64_t phout = (ph & mask[b]) != 0u, mhout = (mh & mask[b]) != 0u;
+            ph = (ph << 1) | phin;
+            mh = (mh << 1) | mhin;
+            P[b] = mh | ~(xv | ph);
+            M[b] = ph & xv;
+            score[b] += (int64_t)phout - (int64_t)mhout;
+            phin = phout;
+            mhin = mhout;
+        }
+        if (score[blocks - 1] < min_score) {
+            min_score = score[blocks - 1];
+            min_col = j;
+        }
+    }
+    /* the reference's convention when no column goes below m: (m, 0) */
+    if (min_score >= (int64_t)m) {
+        min_score = m;
+        min_col = 0;
+    }
+    res->column = min_col;
+    res->score = (uint32_t)min_score;
+}
+
+void oracle_bpm64_batch(const uint32_t *ref_words, uint64_t ref_num_words, uint64_t ref_size,
+                        const oracle_peq_entry_t *peq, const oracle_qinfo_t *qinfo,
+                        const oracle_cand_t *cands, uint32_t num_cands, float distance, int mode,
+                        oracle_result_t *results)
+{
+    uint32_t max_m = 0;
+    for (uint32_t i = 0; i < num_cands; i++)
+        if (qinfo[cands[i].query].size > max_m) max_m = qinfo[cands[i].query].size;
+    const uint32_t max_blocks = (max_m + 63u) / 64u + 1u;
+    #pragma omp parallel
+    {
+        uint64_t *work = (uint64_t *)malloc(sizeof(uint64_t) * 9u * max_blocks);
+        #pragma omp for schedule(static)
+        for (uint32_t i = 0; i < num_cands; i++) {
+            const oracle_qinfo_t qi = qinfo[cands[i].query];
+            verify_candidate_bpm64(ref_words, ref_num_words, ref_size, peq + qi.pos_entry, qi.size,
+                                   cands[i].position, oracle_window_len(qi.size, distance), mode, work,
+                                   &results[i]);
+        }
+        free(work);
+    }
+}
+
 void oracle_myers_batch(const uint32_t *ref_words, uint64_t ref_num_words, uint64_t ref_size,
                         const oracle_peq_entry_t *peq, const oracle_qinfo_t *qinfo,
                         const oracle_cand_t *cands, uint32_t num_cands, float distance,

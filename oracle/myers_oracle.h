@@ -20,6 +20,8 @@
  *                         all-mismatch) and the KAT
  *   oracle_pack_reference golden .4bits.ref written by src/genReference-4bits.c
  *   oracle_build_peq      golden .gem.qry written by src-api/genCandidate-reorder.c
+ *   oracle_bpm64_batch    equal to oracle_myers_batch on the golden inputs and
+ *                         on random batches (two formulations of one DP)
  *   oracle_threshold      thresholds read back from src/checksum-2.c's own
  *                         counts (tests/golden/threshold_kat.json)
  */
@@ -39,6 +41,15 @@ typedef struct { uint32_t pos_entry; uint32_t size; } oracle_qinfo_t;       /* 8
 typedef struct { uint32_t column; uint32_t score; } oracle_result_t;        /* 8 B  */
 
 enum { ORACLE_MODE_ALIGNED = 0, ORACLE_MODE_EXACT = 1 };
+
+/* The same verification with GEMTools' block BPM over 64-bit words
+ * (GEMTools/src/gt_map_align_bpm_generic.c:26-121): a CPU baseline next to
+ * the reference's u32 code.  No query length limit.  OpenMP over
+ * candidates. */
+void oracle_bpm64_batch(const uint32_t *ref_words, uint64_t ref_num_words, uint64_t ref_size,
+                        const oracle_peq_entry_t *peq, const oracle_qinfo_t *qinfo,
+                        const oracle_cand_t *cands, uint32_t num_cands, float distance, int mode,
+                        oracle_result_t *results);
 
 /* Under-threshold bound (uint32)((float)m * e), the float product of
  * src/checksum-2.c:88 (float distance times uint query size). */

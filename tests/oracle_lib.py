@@ -42,6 +42,8 @@ def load():
     lib.oracle_dp_trace.argtypes = [vp, u32, vp, u32, vp, vp]
     lib.oracle_pack_reference.argtypes = [vp, u64, vp]
     lib.oracle_build_peq.argtypes = [vp, u32, vp]
+    lib.oracle_bpm64_batch.argtypes = [vp, u64, u64, vp, vp, vp, u32, f32, i32, vp]
+    lib.oracle_bpm64_batch.restype = None
     lib.oracle_threshold.argtypes = [u32, f32]
     lib.oracle_threshold.restype = u32
     lib.oracle_apply_threshold.argtypes = [vp, vp, u64, f32, vp]
@@ -70,6 +72,18 @@ def myers_batch(ref_words, ref_size, peq, qinfo, cands, distance, mode=MODE_ALIG
     out = np.zeros((len(cands), 2), dtype=np.uint32)
     load().oracle_myers_batch(_ptr(ref_words), len(ref_words), int(ref_size), _ptr(peq), _ptr(qinfo),
                               _ptr(cands), len(cands), float(distance), mode, max_query_len, _ptr(out))
+    return out
+
+
+def bpm64_batch(ref_words, ref_size, peq, qinfo, cands, distance, mode=MODE_ALIGNED):
+    """GEMTools-style 64-bit block BPM over the same inputs: (n, 2) [column, score]."""
+    ref_words = np.ascontiguousarray(ref_words, dtype=np.uint32)
+    peq = np.ascontiguousarray(peq, dtype=np.uint32)
+    qinfo = np.ascontiguousarray(qinfo, dtype=np.uint32)
+    cands = np.ascontiguousarray(cands, dtype=CAND_DTYPE)
+    out = np.zeros((len(cands), 2), dtype=np.uint32)
+    load().oracle_bpm64_batch(_ptr(ref_words), len(ref_words), int(ref_size), _ptr(peq), _ptr(qinfo),
+                              _ptr(cands), len(cands), float(distance), mode, _ptr(out))
     return out
 
 

@@ -209,3 +209,43 @@ def test_apply_threshold_sentinels():
                                         [0xFFFFFFFF, 0xFFFFFFFF], [0xFFFFFFFF, 0xFFFFFFFF]])
     out = O.apply_threshold(qinfo, cands, res, 0.7)
     np.testing.assert_array_equal(out[3:], [[3, 7], [0xFFFFFFFF, 0xFFFFFFFF]])
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
+def test_bpm64_baseline_matches_reference_programs(path):
+    """The 64-bit block BPM baseline (GEMTools formulation) gives the reference's results."""
+    z, peq, qinfo, cands = golden_batch(path)
+    got = O.bpm64_batch(z["ref_words"], z["ref_size"], peq, qinfo, cands, float(z["distance"]), int(z["mode"]))
+    keep = ~z["reads_past_end"]
+    limit = int(z["max_query_len"])
+    if limit:                                      # the API twin skips queries over 4096 bases
+        keep &= qinfo[cands["query"], 1] <= limit
+    np.testing.assert_array_equal(got[keep], z["results"][keep])
+
+
+@pytest.mark.parametrize("mode", [O.MODE_ALIGNED, O.MODE_EXACT])
+def test_bpm64_equals_oracle_random(mode):
+    """Two formulations of one DP (u32 blocks with int8 carry, u64 blocks with
+    1-bit hin) agree on random batches with every length residue mod 64."""
+    rng = np.random.default_rng(70 + mode)
+    size = 200_000
+    codes = rng.integers(0, 5, size=size).astype(np.uint8)
+    words = O.pack_reference(codes)
+    lengths = list(range(1, 200)) + [255, 256, 257, 511, 512, 513, 1000, 1023, 1024, 1025, 3000]
+    peqs, qinfo, off = [], [], 0
+    for m in lengths:
+        start = int(rng.integers(0, size - m))
+        q = codes[start:start + m].copy()
+        q[rng.random(m) < 0.1] = rng.integers(0, 5)
+        p = O.build_peq(q)
+        peqs.append(p)
+        qinfo.append([off, m])
+        off += len(p)
+    peq = np.concatenate(peqs)
+    qinfo = np.array(qinfo, dtype=np.uint32)
+    cq = np.repeat(np.arange(len(lengths)), 3)
+    cp = rng.integers(0, size, size=len(cq))
+    cands = O.make_cands(cq, cp)
+    a = O.myers_batch(words, size, peq, qinfo, cands, 0.1, mode)
+    b = O.bpm64_batch(words, size, peq, qinfo, cands, 0.1, mode)
+    np.testing.assert_array_equal(a, b)

@@ -131,61 +131,11 @@ __device__ __forceinline__ void stage_peq(uint4 *lds, const uint2 *qtab, uint32_
     }
 }
 
-template <int L, int W>
-__device__ __forceinline__ void load_eq(const uint4 *lds_lane, uint32_t row_off, uint32_t (&eq)[W]) {
-    using S = Shape<L, W>;
-    const uint4 *p = lds_lane + row_off;
-#pragma unroll
-    for (int g = 0; g < S::kGroups; g++) {
-        uint4 v = p[g * S::kRows * S::kRowU4];
-        eq[4 * g] = v.x;
-        if (4 * g + 1 < W) eq[4 * g + 1] = v.y;
-        if (4 * g + 2 < W) eq[4 * g + 2] = v.z;
-        if (4 * g + 3 < W) eq[4 * g + 3] = v.w;
-    }
-}
-
 // Truth tables of v_bitop3_b32(a, b, c): bit (4a+2b+c) of the immediate is
 // f(a, b, c); with a = 0xF0, b = 0xCC, c = 0xAA the table is f applied
 // bitwise to those constants.
 constexpr uint32_t kXorOr = (0xF0u ^ 0xCCu) | 0xAAu;          // (a ^ b) | c
 constexpr uint32_t kOrNor = (0xF0u | ~(0xCCu | 0xAAu)) & 0xFFu;  // a | ~(b | c)
-
-// One column for one lane's W words (src/myers-4bits.c:117-134 with a
-// 32W-bit block):
-//   Xv = Eq|Mv;  Eq |= [hin = -1];  Xh = (((Eq&Pv)+Pv)^Pv)|Eq
-//   Ph = Mv|~(Xh|Pv);  Mh = Pv&Xh;  hout = top bits of Ph, Mh
-//   Ph = Ph<<1 | [hin = +1];  Mh = Mh<<1 | [hin = -1];  Pv = Mh|~(Xv|Ph);  Mv = Ph&Xv
-// ph_in / mh_in carry hin in bit 31 (the lower lane's top Ph / Mh word; 0
-// for the bottom lane: D[0][j] = 0).
-template <int W, bool kHin>
-__device__ __forceinline__ void column_step(uint32_t (&Pv)[W], uint32_t (&Mv)[W], const uint32_t (&Eq)[W],
-                                            uint32_t ph_in, uint32_t mh_in, uint32_t &ph_top,
-                                            uint32_t &mh_top) {
-    uint32_t carry = 0, ph_prev = ph_in, mh_prev = mh_in;
-#pragma unroll
-    for (int k = 0; k < W; k++) {
-        const uint32_t pv = Pv[k], mv = Mv[k];
-        const uint32_t xv = Eq[k] | mv;
-        const uint32_t eq = (kHin && k == 0) ? (Eq[0] | (mh_in >> 31)) : Eq[k];
-        uint32_t c_out;
-        const uint32_t sum = __builtin_addc(eq & pv, pv, carry, &c_out);
-        carry = c_out;
-        const uint32_t xh = __builtin_amdgcn_bitop3_b32(sum, pv, eq, kXorOr);
-        const uint32_t ph = __builtin_amdgcn_bitop3_b32(mv, xh, pv, kOrNor);
-        const uint32_t mh = pv & xh;
-        if (k == W - 1) {
-            ph_top = ph;
-            mh_top = mh;
-        }
-        const uint32_t php = (!kHin && k == 0) ? (ph << 1) : __builtin_amdgcn_alignbit(ph, ph_prev, 31u);
-        const uint32_t mhp = (!kHin && k == 0) ? (mh << 1) : __builtin_amdgcn_alignbit(mh, mh_prev, 31u);
-        ph_prev = ph;
-        mh_prev = mh;
-        Pv[k] = __builtin_amdgcn_bitop3_b32(mhp, xv, php, kOrNor);
-        Mv[k] = php & xv;
-    }
-}
 
 // The lower lane's value of the previous step: DPP row_shr:1 (within rows
 // of 16) or wave_shr:1, with 0 on the first lane of every group.
@@ -200,26 +150,28 @@ __device__ __forceinline__ uint32_t from_lower_lane(uint32_t v, uint32_t not_fir
     return v;
 }
 
-// One column for L = 1.  Eq words stream from LDS one group of four words
-// ahead of use (group g+1 is read while group g computes; the caller reads
-// the next column's group 0 at the start of the column), so 12 Eq registers
-// are live instead of 2W.
+// One column for one lane's W words.  Eq words stream from LDS one group of
+// four words ahead of use (group g+1 is read while group g computes; the
+// caller reads the next column's group 0 at the start of the column), so 12
+// Eq registers are live instead of 2W.  For L > 1, ph_in / mh_in carry the
+// lower lane's hin in bit 31 (src/myers-4bits.c:115-134 block form).  The
+// top word's Ph / Mh (before the shift) come back in ph_top / mh_top.
 //
-// kCut: Ukkonen cut-off (Myers 1999, blocks model; GEMTools
+// kCut (L = 1): Ukkonen cut-off (Myers 1999, blocks model; GEMTools
 // gt_map_align_bpm.c:141-214): only the first ga groups are advanced.  ga is
 // the same in every lane, so a skipped group costs no issue slots; the group
 // test is made on a VGPR copy of ga (ga_v), which compiles to an exec mask
-// and a skip branch that keep every word in its register.  S is the lane's D
-// at the top row of group ga-1 (rows of the top-justified layout); without
-// kCut (all groups) S is the score row's D.  Either way the horizontal delta
-// leaving that row updates S.
-template <int W, bool kCut>
-__device__ __forceinline__ void column_l1(uint32_t (&Pv)[W], uint32_t (&Mv)[W], const uint4 &e0,
-                                          const uint4 *lds_lane, uint32_t cur_off, uint32_t ga_v, int32_t &S) {
-    using Sh = Shape<1, W>;
+// and a skip branch that keep every word in its register.  ph_top / mh_top
+// are then those of group ga-1's top word.
+template <int L, int W, bool kCut>
+__device__ __forceinline__ void column_grp(uint32_t (&Pv)[W], uint32_t (&Mv)[W], const uint4 &e0,
+                                           const uint4 *lds_lane, uint32_t cur_off, uint32_t ga_v,
+                                           uint32_t ph_in, uint32_t mh_in, uint32_t &ph_top, uint32_t &mh_top) {
+    using Sh = Shape<L, W>;
+    constexpr bool kHin = L > 1;
     constexpr int kG = Sh::kGroups;
     constexpr uint32_t kGS = (uint32_t)(Sh::kRows * Sh::kRowU4);     // uint4 between groups
-    uint32_t carry = 0, ph_prev = 0, mh_prev = 0;
+    uint32_t carry = 0, ph_prev = ph_in, mh_prev = mh_in;
     uint4 cur = e0, nxt = e0;
 #pragma unroll
     for (int g = 0; g < kG; g++) {
@@ -229,16 +181,16 @@ __device__ __forceinline__ void column_l1(uint32_t (&Pv)[W], uint32_t (&Mv)[W], 
 #pragma unroll
             for (int k = 4 * g; k < 4 * g + 4 && k < W; k++) {
                 const uint32_t pv = Pv[k], mv = Mv[k];
-                const uint32_t eq = e4[k - 4 * g];
-                const uint32_t xv = eq | mv;
+                const uint32_t xv = e4[k - 4 * g] | mv;
+                const uint32_t eq = (kHin && k == 0) ? (e4[0] | (mh_in >> 31)) : e4[k - 4 * g];
                 uint32_t c_out;
                 const uint32_t sum = __builtin_addc(eq & pv, pv, carry, &c_out);
                 carry = c_out;
                 const uint32_t xh = __builtin_amdgcn_bitop3_b32(sum, pv, eq, kXorOr);
                 const uint32_t ph = __builtin_amdgcn_bitop3_b32(mv, xh, pv, kOrNor);
                 const uint32_t mh = pv & xh;
-                const uint32_t php = k == 0 ? (ph << 1) : __builtin_amdgcn_alignbit(ph, ph_prev, 31u);
-                const uint32_t mhp = k == 0 ? (mh << 1) : __builtin_amdgcn_alignbit(mh, mh_prev, 31u);
+                const uint32_t php = (!kHin && k == 0) ? (ph << 1) : __builtin_amdgcn_alignbit(ph, ph_prev, 31u);
+                const uint32_t mhp = (!kHin && k == 0) ? (mh << 1) : __builtin_amdgcn_alignbit(mh, mh_prev, 31u);
                 ph_prev = ph;
                 mh_prev = mh;
                 Pv[k] = __builtin_amdgcn_bitop3_b32(mhp, xv, php, kOrNor);
@@ -247,7 +199,8 @@ __device__ __forceinline__ void column_l1(uint32_t (&Pv)[W], uint32_t (&Mv)[W], 
             cur = nxt;
         }
     }
-    S += (int32_t)(ph_prev >> 31) - (int32_t)(mh_prev >> 31);
+    ph_top = ph_prev;
+    mh_top = mh_prev;
 }
 
 template <int L, int W>
@@ -401,14 +354,8 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
             uint32_t tw3 = text_word(3);
             uint32_t text = text_block(tw0, tw1, 0);
             uint32_t text_next = text_block(tw1, tw2, 1);
-            uint32_t eq[W];
-            uint4 eq4 = make_uint4(0u, 0u, 0u, 0u);
             uint32_t cur_off = (text & 7u) * S::kRowU4;
-            if constexpr (L == 1) {
-                eq4 = lds_lane[cur_off];
-            } else {
-                load_eq<L, W>(lds_lane, cur_off, eq);
-            }
+            uint4 eq4 = lds_lane[cur_off];
             for (uint32_t b = 0; b < nblocks; b++) {
                 const uint32_t tw4 = text_word((int32_t)b + 4);
                 if constexpr (kCut) {
@@ -455,8 +402,9 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
                     const uint32_t next_off = cn * S::kRowU4;
                     if constexpr (L == 1) {
                         const uint4 eq4n = lds_lane[next_off];        // next column, group 0
-                        column_l1<W, kCut>(Pv, Mv, eq4, lds_lane, cur_off, ga_v, score);
+                        column_grp<1, W, kCut>(Pv, Mv, eq4, lds_lane, cur_off, ga_v, 0u, 0u, ph_top, mh_top);
                         eq4 = eq4n;
+                        score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
                         const uint32_t col = 8u * b + (uint32_t)i;
                         const bool on = col < ncols;
                         if constexpr (!kCut) {
@@ -475,11 +423,11 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
                         }
                         __builtin_amdgcn_sched_barrier(0);
                     } else {
-                        uint32_t eqn[W];
-                        load_eq<L, W>(lds_lane, next_off, eqn);
+                        const uint4 eq4n = lds_lane[next_off];        // next column, group 0
                         const uint32_t ph_in = from_lower_lane<L>(ph_top, not_first);
                         const uint32_t mh_in = from_lower_lane<L>(mh_top, not_first);
-                        column_step<W, true>(Pv, Mv, eq, ph_in, mh_in, ph_top, mh_top);
+                        column_grp<L, W, false>(Pv, Mv, eq4, lds_lane, cur_off, 0u, ph_in, mh_in, ph_top, mh_top);
+                        eq4 = eq4n;
                         // score row and first minimum (src-api/coreMyersGPU-CC35.cu:205-207,227-228);
                         // only the group's top lane keeps them, at column t - (L-1)
                         score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
@@ -488,8 +436,6 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
                         min_col = better ? col : min_col;
                         min_score = better ? score : min_score;
                         __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                        for (int k = 0; k < W; k++) eq[k] = eqn[k];
                     }
                     cur_off = next_off;
                 }
@@ -510,9 +456,16 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
     }
 }
 
+// Waves per SIMD the register allocator must leave room for (1 = no limit).
+// The wide one-lane kernels fit 3 (<= 168 VGPRs) or 4 (<= 128) without
+// spilling; more resident waves keep the double-rate logic ops issuing.
+template <int L, int W, bool kCut, bool kPersistent>
+constexpr int kWavesPerEU = (L != 1 || kCut) ? 1 : (W >= 28 ? 3 : ((W >= 20 && !kPersistent) ? 4 : 1));
+
 // Plain launch: a fixed grid strides over the chunks of one slot range.
 template <int L, int W, bool kCut>
-__global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<L, W, kCut, false>)))
+void verify_kernel(myers_verify_args_t a) {
     using S = Shape<L, W>;
     __shared__ Workspace<L, W> ws;
     uint32_t start = a.start, count = a.count;
@@ -534,7 +487,8 @@ __global__ __launch_bounds__(64) void verify_kernel(myers_verify_args_t a) {
 // chunks from the variant's device queue (an atomic head reset by the
 // binning) until the bin is drained; every wave reaches the exit test.
 template <int L, int W, bool kCut>
-__global__ __launch_bounds__(64) void verify_persistent_kernel(myers_verify_args_t a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<L, W, kCut, true>)))
+void verify_persistent_kernel(myers_verify_args_t a) {
     using S = Shape<L, W>;
     __shared__ Workspace<L, W> ws;
     __shared__ uint32_t item_sh;

@@ -195,6 +195,10 @@ typedef struct {
     uint64_t hbmBudgetBytes;    /* admission budget (0 = 200e9 bytes) */
     uint32_t pollMicros;        /* completion poll period while batches run (0 = 20) */
     uint32_t maxBatchCandidates;/* cap per batch (0 = slot capacity) */
+    uint64_t maxBatchCells;     /* cap per batch on the sum of m * n over its candidates, so
+                                   that batches stay short and several run at once
+                                   (0 = 2.5e11; UINT64_MAX = no cap); a request larger than
+                                   the cap runs as a batch of its own */
 } myersServeConfig_t;
 
 typedef struct {

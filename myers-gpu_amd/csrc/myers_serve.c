@@ -38,6 +38,7 @@ typedef struct req {
     uint64_t       t_submit_ns;
     uint64_t       t_done_ns;
     uint64_t       bytes;
+    uint64_t       cells;
     int            state;
     int            error;
     struct req    *next;
@@ -79,6 +80,23 @@ static uint64_t now_ns(void)
 
 /* Device bytes a request occupies while admitted: Peq, qinfo, candidates,
  * results and the binning slot of each candidate. */
+/* About 2.5 ms of verification at 100 TCUPS: short enough that several
+ * batches are in flight and a request does not wait behind a long one. */
+#define DEFAULT_BATCH_CELLS 250000000000ull
+
+/* Sum of m * n over the request's candidates (nominal windows). */
+static uint64_t request_cells(const myersRequest_t *r, float distance)
+{
+    volatile float factor = 1.0f + 2.0f * distance;
+    uint64_t cells = 0;
+    for (uint32_t i = 0; i < r->numCandidates; i++) {
+        const uint32_t m = r->qinfo[r->cands[i].query].size;
+        volatile float n = (float)m * factor;
+        cells += (uint64_t)m * (uint64_t)(uint32_t)n;
+    }
+    return cells;
+}
+
 static uint64_t request_bytes(const myersRequest_t *r)
 {
     return (uint64_t)r->numPEQEntries * sizeof(qryEntry_t) + (uint64_t)r->numQueries * sizeof(qryInfo_t) +
@@ -93,6 +111,7 @@ void myersServeDefaultConfig(myersServeConfig_t *cfg)
     cfg->flags = 0;
     cfg->hbmBudgetBytes = 200000000000ull;
     cfg->pollMicros = 20;
+    cfg->maxBatchCells = DEFAULT_BATCH_CELLS;
 }
 
 /* Copy finished batches' results out; caller holds the lock. */
@@ -135,6 +154,7 @@ static int launch_batch(server_t *sv, slot_t *s)
     qryInfo_t *qinfo = getPEQInfoBuffer(s->buf);
     candInfo_t *cands = getCandidatesBuffer(s->buf);
     uint32_t ne = 0, nq = 0, nc = 0;
+    uint64_t cells = 0;
     s->nreq = 0;
     s->bytes = 0;
     const uint32_t cap_c = (sv->cfg.maxBatchCandidates && sv->cfg.maxBatchCandidates < s->max_cands)
@@ -155,6 +175,7 @@ static int launch_batch(server_t *sv, slot_t *s)
         if (ne + r->numPEQEntries > s->max_entries || nq + r->numQueries > s->max_queries ||
             nc + r->numCandidates > cap_c)
             break;
+        if (s->nreq > 0 && cells + q->cells > sv->cfg.maxBatchCells) break;
         if (sv->inflight_bytes + q->bytes > sv->cfg.hbmBudgetBytes && sv->inflight_bytes > 0) break;
         if (s->nreq == s->cap_req) {
             uint32_t cap = s->cap_req ? 2u * s->cap_req : 64u;
@@ -186,6 +207,7 @@ static int launch_batch(server_t *sv, slot_t *s)
         ne += r->numPEQEntries;
         nq += r->numQueries;
         nc += r->numCandidates;
+        cells += q->cells;
         s->bytes += q->bytes;
         sv->inflight_bytes += q->bytes;
         q->state = REQ_ADMITTED;
@@ -243,7 +265,13 @@ int myersServeStart(void **server, void **myersBuffers, uint32_t numBuffers, con
     server_t *sv = (server_t *)calloc(1, sizeof(server_t));
     if (sv == NULL) return MYERS_E_ALLOCATE_MEM;
     myersServeDefaultConfig(&sv->cfg);
-    if (cfg) sv->cfg = *cfg;
+    if (cfg) {
+        /* callers built against an older, shorter config keep the defaults of the new fields */
+        const size_t n = cfg->structSize && cfg->structSize < sizeof(*cfg) ? cfg->structSize : sizeof(*cfg);
+        memcpy(&sv->cfg, cfg, n);
+        sv->cfg.structSize = sizeof(sv->cfg);
+    }
+    if (sv->cfg.maxBatchCells == 0) sv->cfg.maxBatchCells = DEFAULT_BATCH_CELLS;
     if (sv->cfg.hbmBudgetBytes == 0) sv->cfg.hbmBudgetBytes = 200000000000ull;
     if (sv->cfg.pollMicros == 0) sv->cfg.pollMicros = 20;
     sv->stats.structSize = sizeof(sv->stats);
@@ -289,6 +317,7 @@ int myersServeSubmit(void *server, const myersRequest_t *req, uint64_t *ticket)
     q->t_submit_ns = now_ns();
     q->r = *req;
     q->bytes = request_bytes(req);
+    q->cells = request_cells(req, sv->cfg.distance);
     pthread_mutex_lock(&sv->mu);
     if (sv->stopping) {
         pthread_mutex_unlock(&sv->mu);

@@ -67,7 +67,7 @@ class VariantInfo(ctypes.Structure):
 class ServeConfig(ctypes.Structure):
     _fields_ = [("structSize", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("distance", ctypes.c_float),
                 ("flags", ctypes.c_uint32), ("hbmBudgetBytes", ctypes.c_uint64), ("pollMicros", ctypes.c_uint32),
-                ("maxBatchCandidates", ctypes.c_uint32)]
+                ("maxBatchCandidates", ctypes.c_uint32), ("maxBatchCells", ctypes.c_uint64)]
 
 
 class Request(ctypes.Structure):
@@ -554,7 +554,8 @@ def synchronize(m, i=0):
 class Server:
     """Continuous-batching server over all buffers of a Myers instance."""
 
-    def __init__(self, myers, distance=0.1, flags=0, hbm_budget=0, poll_us=0, max_batch_candidates=0):
+    def __init__(self, myers, distance=0.1, flags=0, hbm_budget=0, poll_us=0, max_batch_candidates=0,
+                 max_batch_cells=0):
         cfg = ServeConfig()
         lib().myersServeDefaultConfig(ctypes.byref(cfg))
         cfg.distance = distance
@@ -564,6 +565,8 @@ class Server:
         if poll_us:
             cfg.pollMicros = poll_us
         cfg.maxBatchCandidates = max_batch_candidates
+        if max_batch_cells:
+            cfg.maxBatchCells = max_batch_cells
         self._h = ctypes.c_void_p()
         check(lib().myersServeStart(ctypes.byref(self._h), myers._handles, myers.num_buffers, ctypes.byref(cfg)),
               "myersServeStart")

@@ -21,6 +21,12 @@ import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "myers-gpu_amd", "python"))
+# The serving leg runs several buffers at once, each with a main stream and
+# side streams; HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by
+# default), and kernels of streams that share a queue run one after another.
+# Give the process enough queues (at least 16) before the runtime starts.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 import numpy as np  # noqa: E402
 

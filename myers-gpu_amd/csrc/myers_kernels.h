@@ -15,13 +15,15 @@ extern "C" {
 enum { MYERS_MAX_VARIANTS = 64 };
 
 /* Kernel variant: one query bit-vector of L*W u32 words split across L
- * lanes of a wave64, W words per lane. */
+ * lanes, W words per lane: L <= 64 lanes of one wave64, or L/64 waves of one
+ * workgroup for L > 64. */
 typedef struct {
     uint32_t lanes;          /* L: lanes per candidate (1..64, power of two) */
     uint32_t words;          /* W: u32 words per lane */
     uint32_t total_words;    /* L*W: longest query = 32*L*W bases */
-    uint32_t slots;          /* candidates per wave = 64 / L */
-    uint32_t lds_bytes;      /* LDS per workgroup (one wave) */
+    uint32_t slots;          /* candidates per workgroup: 64 / L, 1 for L > 64 */
+    uint32_t lds_bytes;      /* LDS per workgroup */
+    uint32_t block;          /* threads per workgroup: 64, or L for L > 64 (L/64 waves) */
 } myers_variant_desc_t;
 
 typedef struct {

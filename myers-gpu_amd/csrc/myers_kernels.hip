@@ -163,14 +163,11 @@ __device__ __forceinline__ uint32_t from_lower_lane(uint32_t v, uint32_t not_fir
 // test is made on a VGPR copy of ga (ga_v), which compiles to an exec mask
 // and a skip branch that keep every word in its register.  ph_top / mh_top
 // are then those of group ga-1's top word.
-template <int L, int W, bool kCut>
+template <int W, bool kHin, bool kCut, uint32_t kGS>
 __device__ __forceinline__ void column_grp(uint32_t (&Pv)[W], uint32_t (&Mv)[W], const uint4 &e0,
                                            const uint4 *lds_lane, uint32_t cur_off, uint32_t ga_v,
                                            uint32_t ph_in, uint32_t mh_in, uint32_t &ph_top, uint32_t &mh_top) {
-    using Sh = Shape<L, W>;
-    constexpr bool kHin = L > 1;
-    constexpr int kG = Sh::kGroups;
-    constexpr uint32_t kGS = (uint32_t)(Sh::kRows * Sh::kRowU4);     // uint4 between groups
+    constexpr int kG = (W + 3) / 4;     // kGS: uint4 between groups in LDS
     uint32_t carry = 0, ph_prev = ph_in, mh_prev = mh_in;
     uint4 cur = e0, nxt = e0;
 #pragma unroll
@@ -402,7 +399,8 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
                     const uint32_t next_off = cn * S::kRowU4;
                     if constexpr (L == 1) {
                         const uint4 eq4n = lds_lane[next_off];        // next column, group 0
-                        column_grp<1, W, kCut>(Pv, Mv, eq4, lds_lane, cur_off, ga_v, 0u, 0u, ph_top, mh_top);
+                        column_grp<W, false, kCut, (uint32_t)(S::kRows * S::kRowU4)>(
+                            Pv, Mv, eq4, lds_lane, cur_off, ga_v, 0u, 0u, ph_top, mh_top);
                         eq4 = eq4n;
                         score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
                         const uint32_t col = 8u * b + (uint32_t)i;
@@ -426,7 +424,8 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
                         const uint4 eq4n = lds_lane[next_off];        // next column, group 0
                         const uint32_t ph_in = from_lower_lane<L>(ph_top, not_first);
                         const uint32_t mh_in = from_lower_lane<L>(mh_top, not_first);
-                        column_grp<L, W, false>(Pv, Mv, eq4, lds_lane, cur_off, 0u, ph_in, mh_in, ph_top, mh_top);
+                        column_grp<W, true, false, (uint32_t)(S::kRows * S::kRowU4)>(
+                            Pv, Mv, eq4, lds_lane, cur_off, 0u, ph_in, mh_in, ph_top, mh_top);
                         eq4 = eq4n;
                         // score row and first minimum (src-api/coreMyersGPU-CC35.cu:205-207,227-228);
                         // only the group's top lane keeps them, at column t - (L-1)
@@ -504,6 +503,250 @@ void verify_persistent_kernel(myers_verify_args_t a) {
         const uint32_t lo = __builtin_amdgcn_readfirstlane(item_sh);
         if (lo >= nchunks) break;
         verify_chunks<L, W, kCut>(a, start, count, lo, min(lo + cf, nchunks), 1u, ws);
+    }
+}
+
+// ------------------------------------------------- multi-wave candidates
+// A wave issues one instruction every few clocks, so the time of one long
+// candidate grows with the steps times the words each lane carries: a
+// 20,000-base read on one wave (64 lanes x 10 words) takes ~5-10 ms.  For
+// latency, a candidate can span Q waves of one workgroup (64Q lanes, W words
+// each).  Lanes run diagonally as in one wave (lane d computes column t - d);
+// at a wave boundary the bottom lane's hin is the top lane of the wave below,
+// passed through LDS: wave q runs kWaveLag steps behind wave q-1 on top of the
+// per-lane lag, each wave's top lane packs its (Ph, Mh) top bits of a block
+// of 8 steps into one word of an LDS ring, and one barrier per block orders
+// writes and reads.  Every wave runs the same number of blocks.
+
+constexpr uint32_t kWaveLag = 8;
+constexpr uint32_t kRingWords = 4;
+
+template <int Q, int W>
+struct WideShape {
+    static constexpr int kLanes = 64 * Q;               // lanes per candidate
+    static constexpr int kGroups = (W + 3) / 4;
+    static constexpr int kTotal = kLanes * W;           // words per candidate
+    static constexpr int kRows = 6;                     // A C G T N + zero row
+    static constexpr int kRowU4 = kLanes;
+    static constexpr int kLdsU4 = kGroups * kRows * kRowU4;
+    static constexpr uint32_t kMaxLag = (uint32_t)kLanes - 1u + kWaveLag * (uint32_t)(Q - 1);
+};
+
+template <int Q, int W>
+struct WideWorkspace {
+    uint4 lds[WideShape<Q, W>::kLdsU4];
+    uint32_t ring[Q - 1][kRingWords];
+};
+
+// Peq of one read, top-justified, transposed for 64Q lanes (the layout of
+// stage_peq with one slot).
+template <int Q, int W>
+__device__ __forceinline__ void stage_peq_wide(uint4 *lds, uint2 qi, const uint32_t *__restrict__ peq,
+                                               uint32_t tid) {
+    using S = WideShape<Q, W>;
+    uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
+    const int32_t pad = 32 * S::kTotal - (int32_t)qi.y;
+    const int32_t nq = (int32_t)((qi.y + 31u) >> 5);
+    for (uint32_t idx = tid; idx < 5u * (uint32_t)S::kTotal; idx += (uint32_t)S::kLanes) {
+        const uint32_t gw = idx % (uint32_t)S::kTotal;
+        const uint32_t c = idx / (uint32_t)S::kTotal;
+        const int32_t b0 = 32 * (int32_t)gw - pad;
+        const int32_t lo = b0 >> 5;
+        const uint32_t sh = (uint32_t)b0 & 31u;
+        const uint32_t *base = peq + (size_t)qi.x * 20u + 4u * c;
+        uint32_t wlo = 0, whi = 0;
+        if (lo >= 0 && lo < nq) wlo = base[(uint32_t)(lo >> 2) * 20u + (uint32_t)(lo & 3)];
+        if (lo + 1 >= 0 && lo + 1 < nq) whi = base[(uint32_t)((lo + 1) >> 2) * 20u + (uint32_t)((lo + 1) & 3)];
+        const uint32_t k = gw % (uint32_t)W, gl = gw / (uint32_t)W;
+        const uint32_t u4 = ((k >> 2) * (uint32_t)S::kRows + c) * (uint32_t)S::kRowU4 + gl;
+        l32[u4 * 4u + (k & 3u)] = (uint32_t)((((uint64_t)whi << 32) | wlo) >> sh);
+    }
+}
+
+// Candidates cand_lo, cand_lo + stride, ... < cand_hi of the slot range
+// [start, start + count), one per workgroup at a time.
+template <int Q, int W>
+__device__ __forceinline__ void verify_wide(const myers_verify_args_t &a, uint32_t start, uint32_t count,
+                                            uint32_t cand_lo, uint32_t cand_hi, uint32_t stride,
+                                            WideWorkspace<Q, W> &ws, uint32_t &staged_query) {
+    using S = WideShape<Q, W>;
+    constexpr uint32_t kGS = (uint32_t)(S::kRows * S::kRowU4);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wq = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave of the workgroup
+    const uint32_t lag = tid + kWaveLag * wq;                      // this lane's column lag
+    const bool bottom = (tid & 63u) == 0u;
+    const uint32_t *__restrict__ ref = a.ref;
+    const int32_t last_word = (int32_t)(a.ref_words - 1);
+    const CandRec *__restrict__ cands = static_cast<const CandRec *>(a.cands);
+    const uint2 *__restrict__ qinfo = reinterpret_cast<const uint2 *>(a.qinfo);
+    uint2 *__restrict__ results = static_cast<uint2 *>(a.results);
+
+    for (uint32_t ci = cand_lo; ci < cand_hi && ci < count; ci += stride) {
+        // every lane reads the same candidate: all values below are uniform
+        const uint32_t cand = a.order ? a.order[start + ci] : start + ci;
+        const CandRec cr = cands[cand];
+        uint32_t m = 0, pos_entry = 0;
+        bool sane = false;
+        if (cr.query < a.num_queries) {
+            const uint2 qi = qinfo[cr.query];
+            pos_entry = qi.x;
+            m = qi.y;
+            sane = m <= 32u * S::kTotal && (uint64_t)pos_entry + ((m + 127u) >> 7) <= (uint64_t)a.num_peq_entries;
+        }
+        if (!sane) m = 0;
+        const uint32_t n = window_len(m, a.distance);
+        const bool live = sane && cr.position < a.ref_size && (a.ref_size - cr.position) > (uint64_t)n;
+        if (!live) {
+            if (tid == 0u) results[cand] = make_uint2(0u, kSkipScore);
+            continue;
+        }
+        uint64_t nib0;
+        uint32_t ncols;
+        if (a.mode == 0) {
+            nib0 = cr.position & ~7ull;
+            ncols = 8u * ((n >> 3) + ((n & 7u) ? 2u : 1u));
+        } else {
+            nib0 = cr.position;
+            ncols = n;
+        }
+        // consecutive candidates of one read keep its Peq in LDS
+        if (cr.query != staged_query) {
+            __syncthreads();
+            stage_peq_wide<Q, W>(ws.lds, make_uint2(pos_entry, m), a.peq, tid);
+            __syncthreads();
+            staged_query = cr.query;
+        }
+        const uint4 *lds_lane = ws.lds + tid;
+        const uint32_t nblocks = (ncols + S::kMaxLag + 7u) >> 3;
+        const int64_t qn = (int64_t)nib0 - (int64_t)lag;
+        const int32_t w0 = (int32_t)(qn >> 3);
+        const uint32_t shift = (uint32_t)(qn & 7) * 4u;
+
+        uint32_t Pv[W], Mv[W];
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            Pv[k] = ~0u;
+            Mv[k] = 0u;
+        }
+        int32_t score = (int32_t)m, min_score = (int32_t)m;
+        uint32_t min_col = 0, ph_top = 0, mh_top = 0;
+        auto text_word = [&](int32_t k) { return ref[min(max(w0 + k, 0), last_word)]; };
+        auto text_block = [&](uint32_t lo, uint32_t hi, uint32_t b) {
+            uint32_t t = sanitize_text(__builtin_amdgcn_alignbit(hi, lo, shift));
+            // steps before this lane's first column read the zero row
+            const int32_t early = (int32_t)lag - 8 * (int32_t)b;
+            if (early > 0) {
+                const uint32_t low = early >= 8 ? ~0u : ((1u << (4 * early)) - 1u);
+                t = (t & ~low) | (0x55555555u & low);
+            }
+            return t;
+        };
+        const uint32_t tw0 = text_word(0);
+        const uint32_t tw1 = text_word(1);
+        uint32_t tw2 = text_word(2);
+        uint32_t tw3 = text_word(3);
+        uint32_t text = text_block(tw0, tw1, 0);
+        uint32_t text_next = text_block(tw1, tw2, 1);
+        uint32_t cur_off = (text & 7u) * (uint32_t)S::kRowU4;
+        uint4 eq4 = lds_lane[cur_off];
+        for (uint32_t b = 0; b < nblocks; b++) {
+            const uint32_t tw4 = text_word((int32_t)b + 4);
+            // hin bits of this block's 8 steps from the wave below: its top
+            // lane's steps 8b-9 .. 8b-2, written in blocks b-2 and b-1
+            uint32_t w1 = 0, w2 = 0;
+            if (Q > 1 && wq > 0u) {
+                if (b >= 1u) w1 = __builtin_amdgcn_readfirstlane(ws.ring[wq - 1u][(b - 1u) % kRingWords]);
+                if (b >= 2u) w2 = __builtin_amdgcn_readfirstlane(ws.ring[wq - 1u][(b - 2u) % kRingWords]);
+            }
+            uint32_t acc = 0;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const uint32_t cn = (i < 7) ? ((text >> (4 * (i + 1))) & 7u) : (text_next & 7u);
+                const uint32_t next_off = cn * (uint32_t)S::kRowU4;
+                const uint4 eq4n = lds_lane[next_off];
+                uint32_t ph_in = from_lower_lane<64>(ph_top, ~0u);
+                uint32_t mh_in = from_lower_lane<64>(mh_top, ~0u);
+                if (Q > 1 && wq > 0u) {
+                    const uint32_t bits = (i == 0) ? (w2 >> 14) : (w1 >> (2 * (i - 1)));
+                    ph_in = bottom ? (bits << 31) : ph_in;
+                    mh_in = bottom ? ((bits >> 1) << 31) : mh_in;
+                }
+                column_grp<W, true, false, kGS>(Pv, Mv, eq4, lds_lane, cur_off, 0u, ph_in, mh_in, ph_top, mh_top);
+                eq4 = eq4n;
+                if (Q > 1 && wq + 1u < (uint32_t)Q) {
+                    const uint32_t pt = __builtin_amdgcn_readlane(ph_top, 63) >> 31;
+                    const uint32_t mt = __builtin_amdgcn_readlane(mh_top, 63) >> 31;
+                    acc |= (pt | (mt << 1)) << (2 * i);
+                }
+                // score row: the top lane of the top wave, at column t - kMaxLag
+                score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
+                const uint32_t col = 8u * b + (uint32_t)i - S::kMaxLag;
+                const bool better = (score < min_score) && (col < ncols);
+                min_col = better ? col : min_col;
+                min_score = better ? score : min_score;
+                cur_off = next_off;
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (Q > 1) {
+                if (wq + 1u < (uint32_t)Q && bottom) ws.ring[wq][b % kRingWords] = acc;
+                __syncthreads();
+            }
+            text = text_next;
+            text_next = text_block(tw2, tw3, b + 2);
+            tw2 = tw3;
+            tw3 = tw4;
+        }
+        if (tid == (uint32_t)S::kLanes - 1u) {
+            uint2 r = make_uint2(min_col, (uint32_t)min_score);
+            if (a.threshold >= 0.0f && (uint32_t)min_score > threshold_of(m, a.threshold))
+                r = make_uint2(kNoMatch, kNoMatch);
+            results[cand] = r;
+        }
+    }
+}
+
+template <int Q, int W>
+__device__ __forceinline__ void zero_row_init_wide(WideWorkspace<Q, W> &ws) {
+    using S = WideShape<Q, W>;
+    uint32_t *l32 = reinterpret_cast<uint32_t *>(ws.lds);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(S::kGroups * S::kRowU4 * 4); i += (uint32_t)S::kLanes) {
+        const uint32_t u4 = i >> 2, g = u4 / S::kRowU4, r = u4 % S::kRowU4;
+        l32[((g * S::kRows + kZeroRow) * S::kRowU4 + r) * 4u + (i & 3u)] = 0u;
+    }
+    __syncthreads();
+}
+
+template <int Q, int W>
+__global__ __launch_bounds__(64 * Q) void verify_wide_kernel(myers_verify_args_t a) {
+    __shared__ WideWorkspace<Q, W> ws;
+    uint32_t start = a.start, count = a.count;
+    if (a.bin_ranges != nullptr) {
+        start = a.bin_ranges[2 * a.variant];
+        count = a.bin_ranges[2 * a.variant + 1];
+    }
+    zero_row_init_wide<Q, W>(ws);
+    uint32_t first = blockIdx.x;
+    if ((gridDim.x & 7u) == 0u) first = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    uint32_t staged = 0xFFFFFFFFu;
+    verify_wide<Q, W>(a, start, count, first, count, gridDim.x, ws, staged);
+}
+
+template <int Q, int W>
+__global__ __launch_bounds__(64 * Q) void verify_wide_persistent_kernel(myers_verify_args_t a) {
+    __shared__ WideWorkspace<Q, W> ws;
+    __shared__ uint32_t item_sh;
+    const uint32_t start = a.bin_ranges[2 * a.variant];
+    const uint32_t count = a.bin_ranges[2 * a.variant + 1];
+    const uint32_t cf = a.chunk_factor ? a.chunk_factor : 1u;
+    zero_row_init_wide<Q, W>(ws);
+    uint32_t staged = 0xFFFFFFFFu;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) item_sh = atomicAdd(&a.queue_heads[a.variant], cf);
+        __syncthreads();
+        const uint32_t lo = __builtin_amdgcn_readfirstlane(item_sh);
+        if (lo >= count) break;
+        verify_wide<Q, W>(a, start, count, lo, min(lo + cf, count), 1u, ws, staged);
     }
 }
 
@@ -607,6 +850,10 @@ __global__ __launch_bounds__(256) void bin_scatter_kernel(const uint2 *qinfo, ui
     X(2, 8) X(2, 12) X(2, 16) X(4, 8) X(4, 12) X(4, 16) X(8, 8) X(8, 12) X(8, 16)                \
     X(16, 8) X(16, 12) X(16, 16) X(32, 8) X(32, 12) X(32, 16) X(64, 8) X(64, 10) X(64, 12)
 
+// Multi-wave variants (lanes = 64 Q per candidate, one candidate per
+// workgroup of Q waves): long reads with lower per-candidate latency.
+#define MYERS_WIDE_LIST(X) X(2, 4) X(2, 6) X(4, 2) X(4, 3)
+
 namespace {
 struct VariantEntry {
     uint32_t lanes, words;
@@ -614,6 +861,7 @@ struct VariantEntry {
     const void *persistent_fn;
     const void *cut_fn;              // threshold mode with the cut-off, or null
     const void *cut_persistent_fn;
+    uint32_t block;                  // threads per workgroup
 };
 template <int L, int W>
 constexpr bool kHasCut = L == 1 && W >= 8 && W % 4 == 0;
@@ -629,8 +877,11 @@ constexpr const void *cut_kernel(bool persistent) {
 #define MYERS_VARIANT_ENTRY(L, W)                                                                \
     {L, W, reinterpret_cast<const void *>(&verify_kernel<L, W, false>),                         \
      reinterpret_cast<const void *>(&verify_persistent_kernel<L, W, false>), cut_kernel<L, W>(false), \
-     cut_kernel<L, W>(true)},
-const VariantEntry kVariants[] = {MYERS_VARIANT_LIST(MYERS_VARIANT_ENTRY)};
+     cut_kernel<L, W>(true), 64u},
+#define MYERS_WIDE_ENTRY(Q, W)                                                                   \
+    {64u * (Q), W, reinterpret_cast<const void *>(&verify_wide_kernel<Q, W>),                   \
+     reinterpret_cast<const void *>(&verify_wide_persistent_kernel<Q, W>), nullptr, nullptr, 64u * (Q)},
+const VariantEntry kVariants[] = {MYERS_VARIANT_LIST(MYERS_VARIANT_ENTRY) MYERS_WIDE_LIST(MYERS_WIDE_ENTRY)};
 constexpr uint32_t kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 static_assert(kNumVariants <= MYERS_MAX_VARIANTS, "variant table too large");
 }  // namespace
@@ -643,7 +894,8 @@ extern "C" int myers_variant_desc(uint32_t v, myers_variant_desc_t *out) {
     out->lanes = L;
     out->words = W;
     out->total_words = L * W;
-    out->slots = 64u / L;
+    out->slots = L <= 64u ? 64u / L : 1u;
+    out->block = kVariants[v].block;
     hipFuncAttributes attr;
     out->lds_bytes = hipFuncGetAttributes(&attr, kVariants[v].fn) == hipSuccess ? (uint32_t)attr.sharedSizeBytes : 0u;
     return 0;
@@ -652,7 +904,7 @@ extern "C" int myers_variant_desc(uint32_t v, myers_variant_desc_t *out) {
 extern "C" int myers_variant_occupancy(uint32_t v, int *wg_per_cu) {
     if (v >= kNumVariants || wg_per_cu == nullptr) return -1;
     int blocks = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kVariants[v].fn, 64, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kVariants[v].fn, (int)kVariants[v].block, 0);
     if (e != hipSuccess) return (int)e;
     *wg_per_cu = blocks;
     return 0;
@@ -661,7 +913,8 @@ extern "C" int myers_variant_occupancy(uint32_t v, int *wg_per_cu) {
 extern "C" int myers_variant_occupancy_persistent(uint32_t v, int *wg_per_cu) {
     if (v >= kNumVariants || wg_per_cu == nullptr) return -1;
     int blocks = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kVariants[v].persistent_fn, 64, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kVariants[v].persistent_fn,
+                                                                (int)kVariants[v].block, 0);
     if (e != hipSuccess) return (int)e;
     *wg_per_cu = blocks;
     return 0;
@@ -675,7 +928,7 @@ extern "C" int myers_launch_verify(uint32_t v, const myers_verify_args_t *args, 
     const VariantEntry &ve = kVariants[v];
     const bool cut = args->threshold >= 0.0f && ve.cut_fn != nullptr;
     const void *fn = cut ? (persistent ? ve.cut_persistent_fn : ve.cut_fn) : (persistent ? ve.persistent_fn : ve.fn);
-    hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(64), kargs, 0, stream);
+    hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(ve.block), kargs, 0, stream);
     return (int)e;
 }
 

@@ -273,3 +273,25 @@ def test_graph_replays_match_eager(engine):
         assert m.stats().graphCaptures == 1
         for x in (dp, dq, dc, dr):
             x.free()
+
+
+@pytest.mark.parametrize("engine", [M.ENGINE_PLAIN, M.ENGINE_PERSISTENT])
+def test_multiwave_variants_in_mixed_batches(engine):
+    """Long reads on the multi-wave variants (one candidate per workgroup of
+    2 or 4 waves) next to one-wave variants in one binned batch."""
+    announce(f"multi-wave engine {engine}")
+    ref, b = synth_case(35, 3_000_000, 120, choices=[300, 1000, 3000, 7000, 12000, 16000, 20000, 24000],
+                        err_range=(0.01, 0.12), sub_frac=0.5, candidates=3, distance=0.1)
+    with M.Myers(ref, 3_000_000, avg_query=6000, cands_per_query=3, buffer_bytes=256 << 20) as m:
+        variants = m.variants()
+        wide = {(v["lanes"], v["words"]): i for i, v in enumerate(variants) if v["lanes"] > 64}
+        assert set(wide) >= {(128, 4), (128, 6), (256, 2), (256, 3)}
+        table = m.bin_table()
+        w = np.arange(len(table))
+        table[(w > 128) & (w <= 512)] = wide[(256, 2)]
+        table[(w > 512) & (w <= 640)] = wide[(128, 6)]
+        table[w > 640] = wide[(256, 3)]
+        m.set_bin_table(table)
+        for mode in (M.WINDOW_ALIGNED, M.WINDOW_EXACT):
+            got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=mode | engine)
+            np.testing.assert_array_equal(got, oracle_for(ref, 3_000_000, b, 0.1, mode))

@@ -655,8 +655,8 @@ __device__ __forceinline__ void verify_wide(const myers_verify_args_t &a, uint32
             // lane's steps 8b-9 .. 8b-2, written in blocks b-2 and b-1
             uint32_t w1 = 0, w2 = 0;
             if (Q > 1 && wq > 0u) {
-                if (b >= 1u) w1 = __builtin_amdgcn_readfirstlane(ws.ring[wq - 1u][(b - 1u) % kRingWords]);
-                if (b >= 2u) w2 = __builtin_amdgcn_readfirstlane(ws.ring[wq - 1u][(b - 2u) % kRingWords]);
+                if (b >= 1u) w1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)ws.ring[wq - 1u][(b - 1u) % kRingWords]);
+                if (b >= 2u) w2 = (uint32_t)__builtin_amdgcn_readfirstlane((int)ws.ring[wq - 1u][(b - 2u) % kRingWords]);
             }
             uint32_t acc = 0;
 #pragma unroll
@@ -674,8 +674,9 @@ __device__ __forceinline__ void verify_wide(const myers_verify_args_t &a, uint32
                 column_grp<W, true, false, kGS>(Pv, Mv, eq4, lds_lane, cur_off, 0u, ph_in, mh_in, ph_top, mh_top);
                 eq4 = eq4n;
                 if (Q > 1 && wq + 1u < (uint32_t)Q) {
-                    const uint32_t pt = __builtin_amdgcn_readlane(ph_top, 63) >> 31;
-                    const uint32_t mt = __builtin_amdgcn_readlane(mh_top, 63) >> 31;
+                    // (the builtin returns int: shift as unsigned)
+                    const uint32_t pt = (uint32_t)__builtin_amdgcn_readlane((int)ph_top, 63) >> 31;
+                    const uint32_t mt = (uint32_t)__builtin_amdgcn_readlane((int)mh_top, 63) >> 31;
                     acc |= (pt | (mt << 1)) << (2 * i);
                 }
                 // score row: the top lane of the top wave, at column t - kMaxLag

@@ -27,7 +27,11 @@ SHORT = [64, 100, 128, 160, 200, 256, 320, 400, 480, 600, 700, 800, 900, 1000]
 LONG = [1500, 2000, 3000, 4000, 5000, 6500, 8000, 10000, 13000, 16000, 20000, 24576]
 CHUNK_FACTORS = (1, 2, 4)
 REF_SIZE = 200_000_000
-TARGET_CELLS = 1.2e11
+# Each measurement must run in the throughput regime: enough candidates to
+# keep every SIMD of the 256 CUs busy for many rounds (a batch that fills the
+# GPU only once favours variants that spread a read over more lanes).
+TARGET_CELLS = 1.5e12
+MIN_CANDIDATES, MAX_CANDIDATES = 20_000, 2_000_000
 
 
 def log(msg):
@@ -73,16 +77,15 @@ def main():
     ref = M.synth_reference(99, REF_SIZE)
     # start from the built-in table (no tuning file) so earlier tables do not leak in
     os.environ["MYERS_TUNING_FILE"] = os.devnull
-    m = M.Myers(ref, REF_SIZE, avg_query=1000, cands_per_query=11, buffer_bytes=1 << 30)
+    m = M.Myers(ref, REF_SIZE, avg_query=1000, cands_per_query=11, buffer_bytes=3 << 30)
     variants = m.variants()
     default_table = m.bin_table()
     records, chosen = [], []
     for length in lengths:
         w = (length + 31) // 32
         cands_per_read = 11 if length <= 1024 else 4
-        n_cands = int(TARGET_CELLS / (length * 1.2 * length))
-        # enough candidates to fill 256 CUs even at 64 lanes per read
-        n_reads = max(-(-4096 // cands_per_read), min(200_000, n_cands // cands_per_read))
+        n_cands = int(min(MAX_CANDIDATES, max(MIN_CANDIDATES, TARGET_CELLS / (length * 1.2 * length))))
+        n_reads = -(-n_cands // cands_per_read)
         spec = M.make_spec(1000 + length, n_reads, length=length, sub=0.03, ins=0.005, dele=0.005,
                            candidates=cands_per_read, distance=0.1)
         b = M.synth_batch(spec, ref, REF_SIZE)

@@ -24,10 +24,11 @@
 //     delta and the query rows see the semi-global boundary D[0][j]=0.  The
 //     score row m-1 is therefore bit 31 of the top lane's top word for every
 //     m the variant covers.
-//   * Peq masks are transposed into LDS per lane as
-//     [ceil(W/4)][rows][64 lanes] x 128-bit (rows = 5 bases, plus a zero row
-//     for L > 1), so one column's Eq words are ceil(W/4) conflict-free
-//     ds_read_b128 at an address chosen by the column's base code.
+//   * Peq masks are staged into LDS once per distinct read of a wave, as
+//     [ceil(W/4) groups][read slot][rows][L lanes] x 128-bit (rows = 5 bases,
+//     plus a zero row for L > 1), so one column's Eq words are ceil(W/4)
+//     ds_read_b128 at an address chosen by the column's base code, and the
+//     (slot, base) pairs of a wave fall on consecutive 16-byte bank groups.
 //   * The packed reference (8 bases per u32) streams through a register
 //     window per lane, two 8-column blocks ahead of use; exact-start windows
 //     and the diagonal lag use a per-lane funnel shift.
@@ -87,13 +88,16 @@ struct Shape {
     static constexpr int kSlots = 64 / L;               // candidates per wave
     static constexpr int kRows = L > 1 ? 6 : 5;         // A C G T N (+ zero row)
     // Peq is staged once per distinct query of the wave ("query slot"), not
-    // once per candidate; up to kQSlots queries per pass.  For L = 1 an odd
-    // slot stride spreads the five rows of one query over distinct banks.
+    // once per candidate; up to kQSlots queries per pass.  LDS layout
+    // [group][slot][row][lane] x 128 bit: the uint4s one column reads for
+    // the (slot, base) pairs of a wave are consecutive in slot*rows + base,
+    // so they spread evenly over the LDS banks.
     static constexpr int kQSlotsMax = L == 1 ? 16 : 8;
     static constexpr int kQSlots = kSlots < kQSlotsMax ? kSlots : kQSlotsMax;
-    static constexpr int kQStride = kQSlots + (L == 1 ? 1 : 0);
-    static constexpr int kRowU4 = kQStride * L;         // uint4 per (group, row)
-    static constexpr int kLdsU4 = kGroups * kRows * kRowU4;
+    static constexpr int kRowU4 = L;                    // uint4 between rows of a slot
+    static constexpr int kSlotU4 = kRows * L;           // uint4 between slots
+    static constexpr int kGroupU4 = kQSlots * kSlotU4;  // uint4 between groups
+    static constexpr int kLdsU4 = kGroups * kGroupU4;
 };
 
 // Stage the Peq words of the wave's query slots into LDS, top-justified,
@@ -103,7 +107,7 @@ struct Shape {
 // (lo = floor((32gw - pad)/32)) shifted right by (-pad) mod 32.  Query word
 // j (rows 32j..32j+31) of base c is bitmap[c][j%4] of entry posEntry + j/4;
 // words outside the query read 0.  Lane gl = gw / W of the group reads word
-// k = gw % W at LDS uint4 ((k/4)*rows + c)*kRowU4 + slot*L + gl, lane k%4.
+// k = gw % W at LDS uint4 (k/4)*kGroupU4 + slot*kSlotU4 + c*L + gl, lane k%4.
 template <int L, int W>
 __device__ __forceinline__ void stage_peq(uint4 *lds, const uint2 *qtab, uint32_t nslots,
                                           const uint32_t *__restrict__ peq, uint32_t lane) {
@@ -126,7 +130,7 @@ __device__ __forceinline__ void stage_peq(uint4 *lds, const uint2 *qtab, uint32_
         if (lo >= 0 && lo < nq) wlo = base[(uint32_t)(lo >> 2) * 20u + (uint32_t)(lo & 3)];
         if (lo + 1 >= 0 && lo + 1 < nq) whi = base[(uint32_t)((lo + 1) >> 2) * 20u + (uint32_t)((lo + 1) & 3)];
         const uint32_t k = gw % (uint32_t)W, gl = gw / (uint32_t)W;
-        const uint32_t u4 = ((k >> 2) * (uint32_t)S::kRows + c) * (uint32_t)S::kRowU4 + slot * (uint32_t)L + gl;
+        const uint32_t u4 = (k >> 2) * (uint32_t)S::kGroupU4 + slot * (uint32_t)S::kSlotU4 + c * (uint32_t)L + gl;
         l32[u4 * 4u + (k & 3u)] = (uint32_t)((((uint64_t)whi << 32) | wlo) >> sh);
     }
 }
@@ -212,9 +216,12 @@ __device__ __forceinline__ void zero_row_init(Workspace<L, W> &ws) {
     if constexpr (S::kRows > 5) {
         // the all-zero row read by lanes before their first column
         uint32_t *l32 = reinterpret_cast<uint32_t *>(ws.lds);
-        for (uint32_t i = threadIdx.x; i < (uint32_t)(S::kGroups * S::kRowU4 * 4); i += 64u) {
-            const uint32_t u4 = i >> 2, g = u4 / S::kRowU4, r = u4 % S::kRowU4;
-            l32[((g * S::kRows + kZeroRow) * S::kRowU4 + r) * 4u + (i & 3u)] = 0u;
+        constexpr uint32_t kPerGroup = (uint32_t)(S::kQSlots * L);    // (slot, lane) pairs
+        for (uint32_t i = threadIdx.x; i < (uint32_t)S::kGroups * kPerGroup * 4u; i += 64u) {
+            const uint32_t u4 = i >> 2, g = u4 / kPerGroup, r = u4 % kPerGroup;
+            const uint32_t slot = r / (uint32_t)L, gl = r % (uint32_t)L;
+            l32[(g * (uint32_t)S::kGroupU4 + slot * (uint32_t)S::kSlotU4 + kZeroRow * (uint32_t)L + gl) * 4u +
+                (i & 3u)] = 0u;
         }
     }
 }
@@ -294,7 +301,7 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
             __syncthreads();
             stage_peq<L, W>(lds, qtab, min(nqslots - base_slot, (uint32_t)S::kQSlots), a.peq, lane);
             __syncthreads();
-            const uint4 *lds_lane = lds + my_slot * L + gl;
+            const uint4 *lds_lane = lds + my_slot * (uint32_t)S::kSlotU4 + gl;
             const uint32_t ncols = in_pass ? ncols_live : 0u;
 
             // Steps: the top lane of a group finishes column ncols-1 at step
@@ -399,7 +406,7 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
                     const uint32_t next_off = cn * S::kRowU4;
                     if constexpr (L == 1) {
                         const uint4 eq4n = lds_lane[next_off];        // next column, group 0
-                        column_grp<W, false, kCut, (uint32_t)(S::kRows * S::kRowU4)>(
+                        column_grp<W, false, kCut, (uint32_t)S::kGroupU4>(
                             Pv, Mv, eq4, lds_lane, cur_off, ga_v, 0u, 0u, ph_top, mh_top);
                         eq4 = eq4n;
                         score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
@@ -424,7 +431,7 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
                         const uint4 eq4n = lds_lane[next_off];        // next column, group 0
                         const uint32_t ph_in = from_lower_lane<L>(ph_top, not_first);
                         const uint32_t mh_in = from_lower_lane<L>(mh_top, not_first);
-                        column_grp<W, true, false, (uint32_t)(S::kRows * S::kRowU4)>(
+                        column_grp<W, true, false, (uint32_t)S::kGroupU4>(
                             Pv, Mv, eq4, lds_lane, cur_off, 0u, ph_in, mh_in, ph_top, mh_top);
                         eq4 = eq4n;
                         // score row and first minimum (src-api/coreMyersGPU-CC35.cu:205-207,227-228);

@@ -42,7 +42,12 @@
 #define DEFAULT_BUFFER_CAP (2ull << 30)
 #define EV_RING          32u            /* kernel-timing event pairs per buffer */
 #define SIDE_STREAMS     4u             /* fork width of a persistent launch */
-#define WORK_WORDS       (6u * MYERS_MAX_VARIANTS + 1u)
+/* d_work: ranges (2x) | err | queue part heads | binning block counts */
+#define WORK_RANGES      0u
+#define WORK_ERR         (2u * MYERS_MAX_VARIANTS)
+#define WORK_HEADS       (8u * MYERS_MAX_VARIANTS)     /* 2 KB in: 128-byte aligned */
+#define WORK_BLOCKS      (WORK_HEADS + (uint32_t)MYERS_QUEUE_WORDS)
+#define WORK_WORDS       (WORK_BLOCKS + (uint32_t)MYERS_BIN_WORDS)
 
 typedef struct myers_ctx {
     int       device;
@@ -78,7 +83,7 @@ typedef struct myers_buffer {
     resEntry_t  *h_results;
     resEntry_t  *d_results;
     uint32_t    *d_order;
-    uint32_t    *d_work;          /* counts | cursors | ranges(2x) | queue heads | spare | err */
+    uint32_t    *d_work;          /* ranges(2x) | err | queue part heads | binning block counts */
     uint32_t    *h_err;           /* pinned copy of the binning error flag */
     hipEvent_t   ev_batch_start, ev_batch_end;
     /* HIP events around the verification launches of each batch, in a ring
@@ -643,14 +648,12 @@ static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan
         return MYERS_SUCCESS;
     }
     /* device-side binning */
-    uint32_t *counts = b->d_work;
-    uint32_t *cursors = counts + MYERS_MAX_VARIANTS;
-    uint32_t *ranges = cursors + MYERS_MAX_VARIANTS;
-    uint32_t *heads = ranges + 2u * MYERS_MAX_VARIANTS;
-    uint32_t *err = b->d_work + WORK_WORDS - 1u;
+    uint32_t *ranges = b->d_work + WORK_RANGES;
+    uint32_t *heads = b->d_work + WORK_HEADS;
+    uint32_t *err = b->d_work + WORK_ERR;
     rc = myers_launch_binning((const uint32_t *)bt->d_qinfo, bt->num_queries, bt->d_cands, bt->num_cands,
-                              ctx->d_bin_of_words, max_supported_words(ctx), ctx->num_variants, counts, cursors,
-                              ranges, b->d_order, err, bt->d_results, heads, b->stream);
+                              ctx->d_bin_of_words, max_supported_words(ctx), ctx->num_variants,
+                              b->d_work + WORK_BLOCKS, ranges, b->d_order, err, bt->d_results, heads, b->stream);
     if (rc) return hip_fail((hipError_t)rc, "binning launch", __FILE__, __LINE__);
     HIP_TRY(hipMemcpyAsync(b->h_err, err, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
     b->binned = 1;

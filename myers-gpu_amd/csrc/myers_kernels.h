@@ -13,6 +13,10 @@ extern "C" {
 #endif
 
 enum { MYERS_MAX_VARIANTS = 64 };
+/* Persistent work queues: each variant's chunks are split into one part per
+ * XCD, each part with its own head counter on its own 128-byte line. */
+enum { MYERS_QUEUE_PARTS = 8, MYERS_QUEUE_STRIDE = 32 };
+#define MYERS_QUEUE_WORDS (MYERS_MAX_VARIANTS * MYERS_QUEUE_PARTS * MYERS_QUEUE_STRIDE)
 
 /* Kernel variant: one query bit-vector of L*W u32 words split across L
  * lanes, W words per lane: L <= 64 lanes of one wave64, or L/64 waves of one
@@ -46,7 +50,7 @@ typedef struct {
                                     {0xFFFFFFFF, 0xFFFFFFFF} and L = 1 variants cut off rows */
     uint32_t        chunk_factor;  /* persistent launch: chunks per queue pull */
     void           *results;     /* resEntry_t[] in caller order */
-    uint32_t       *queue_heads; /* persistent launch: one atomic head per variant */
+    uint32_t       *queue_heads; /* persistent launch: MYERS_QUEUE_WORDS words of part heads */
 } myers_verify_args_t;
 
 /* Variant table compiled into the library. */
@@ -65,16 +69,20 @@ int myers_launch_verify(uint32_t v, const myers_verify_args_t *args, uint32_t gr
 /* Resident workgroups per CU of the persistent kernel of variant v. */
 int myers_variant_occupancy_persistent(uint32_t v, int *wg_per_cu);
 
-/* Device-side length binning.  bin_of_words[w] (w = ceil(m/32), 0..max_words)
- * gives the variant of a query of w words.  counts/offsets/cursors hold
- * MYERS_MAX_VARIANTS u32 each; ranges gets {start, count} pairs; order gets
- * the candidate of every slot.  A candidate whose query is longer than
- * max_words words gets the skip sentinel in results and sets err (u32) to 1;
- * one naming a query >= num_queries gets the sentinel and sets err to 2. */
+/* Device-side length binning: a stable partition of the candidates by
+ * variant.  bin_of_words[w] (w = ceil(m/32), 0..max_words) gives the variant
+ * of a query of w words.  block_counts holds MYERS_BIN_WORDS u32 of
+ * workspace; ranges gets {start, count} pairs; order gets the candidate of
+ * every slot (input order within each variant); the queue part heads are
+ * reset.  A candidate whose query is longer than max_words words gets the
+ * skip sentinel in results and sets err (u32) to 1; one naming a query >=
+ * num_queries gets the sentinel and sets err to 2. */
+enum { MYERS_BIN_BLOCKS = 4096 };
+#define MYERS_BIN_WORDS (MYERS_BIN_BLOCKS * MYERS_MAX_VARIANTS)
 int myers_launch_binning(const uint32_t *qinfo, uint32_t num_queries, const void *cands, uint32_t num_cands,
                          const uint8_t *bin_of_words, uint32_t max_words, uint32_t num_variants,
-                         uint32_t *counts, uint32_t *cursors, uint32_t *ranges, uint32_t *order,
-                         uint32_t *err, void *results, uint32_t *queue_heads, hipStream_t stream);
+                         uint32_t *block_counts, uint32_t *ranges, uint32_t *order, uint32_t *err,
+                         void *results, uint32_t *queue_heads, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -462,6 +462,37 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
     }
 }
 
+// Take the next range of work items [lo, hi) for a persistent workgroup from
+// the variant's queue: items [0, n) are split into one part per XCD, a
+// workgroup starts on the part of the XCD it runs on (workgroups are
+// dispatched round-robin over the 8 XCDs) and moves on to the other parts
+// when its own is drained, so the atomics of one part stay within one XCD
+// until the tail.  Each pull takes a guided share of what is left in the
+// part (at least min_take items).  Thread 0 only; *part counts the parts
+// this workgroup has found drained.  Returns false when all are.
+__device__ __forceinline__ bool pull_work(uint32_t *heads, uint32_t n, uint32_t min_take, uint32_t &part,
+                                          uint32_t &lo, uint32_t &hi) {
+    const uint32_t xcd = blockIdx.x % (uint32_t)MYERS_QUEUE_PARTS;
+    const uint32_t workers = max(1u, gridDim.x / (uint32_t)MYERS_QUEUE_PARTS);
+    while (part < (uint32_t)MYERS_QUEUE_PARTS) {
+        const uint32_t x = (xcd + part) % (uint32_t)MYERS_QUEUE_PARTS;
+        const uint32_t p_lo = (uint32_t)(((uint64_t)n * x) / MYERS_QUEUE_PARTS);
+        const uint32_t p_hi = (uint32_t)(((uint64_t)n * (x + 1u)) / MYERS_QUEUE_PARTS);
+        uint32_t *head = heads + x * (uint32_t)MYERS_QUEUE_STRIDE;
+        const uint32_t seen = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t left = (p_hi - p_lo) > seen ? (p_hi - p_lo) - seen : 0u;
+        const uint32_t take = max(min_take, left / (2u * workers));
+        const uint32_t got = atomicAdd(head, take);
+        if (got < p_hi - p_lo) {
+            lo = p_lo + got;
+            hi = min(p_hi, lo + take);
+            return true;
+        }
+        part++;
+    }
+    return false;
+}
+
 // Waves per SIMD the register allocator must leave room for (1 = no limit).
 // The wide one-lane kernels fit 3 (<= 168 VGPRs) or 4 (<= 128) without
 // spilling; more resident waves keep the double-rate logic ops issuing.
@@ -489,27 +520,35 @@ void verify_kernel(myers_verify_args_t a) {
     verify_chunks<L, W, kCut>(a, start, count, first, nchunks, gridDim.x, ws);
 }
 
-// Persistent launch: a resident grid pulls work items of `chunk_factor`
-// chunks from the variant's device queue (an atomic head reset by the
+// Persistent launch: a resident grid pulls ranges of chunks (at least
+// `chunk_factor`) from the variant's device queue (part heads reset by the
 // binning) until the bin is drained; every wave reaches the exit test.
 template <int L, int W, bool kCut>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<L, W, kCut, true>)))
 void verify_persistent_kernel(myers_verify_args_t a) {
     using S = Shape<L, W>;
     __shared__ Workspace<L, W> ws;
-    __shared__ uint32_t item_sh;
+    __shared__ uint32_t item_sh, item_hi_sh;
     const uint32_t start = a.bin_ranges[2 * a.variant];
     const uint32_t count = a.bin_ranges[2 * a.variant + 1];
     const uint32_t nchunks = (count + S::kSlots - 1) / S::kSlots;
     const uint32_t cf = a.chunk_factor ? a.chunk_factor : 1u;
+    uint32_t *heads = a.queue_heads + a.variant * (uint32_t)(MYERS_QUEUE_PARTS * MYERS_QUEUE_STRIDE);
+    uint32_t part = 0;
     zero_row_init<L, W>(ws);
     for (;;) {
         __syncthreads();
-        if (threadIdx.x == 0) item_sh = atomicAdd(&a.queue_heads[a.variant], cf);
+        if (threadIdx.x == 0) {
+            uint32_t lo = 0, hi = 0;
+            if (!pull_work(heads, nchunks, cf, part, lo, hi)) lo = hi = 0xFFFFFFFFu;
+            item_sh = lo;
+            item_hi_sh = hi;
+        }
         __syncthreads();
         const uint32_t lo = __builtin_amdgcn_readfirstlane(item_sh);
-        if (lo >= nchunks) break;
-        verify_chunks<L, W, kCut>(a, start, count, lo, min(lo + cf, nchunks), 1u, ws);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane(item_hi_sh);
+        if (lo == 0xFFFFFFFFu) break;
+        verify_chunks<L, W, kCut>(a, start, count, lo, hi, 1u, ws);
     }
 }
 
@@ -742,26 +781,33 @@ __global__ __launch_bounds__(64 * Q) void verify_wide_kernel(myers_verify_args_t
 template <int Q, int W>
 __global__ __launch_bounds__(64 * Q) void verify_wide_persistent_kernel(myers_verify_args_t a) {
     __shared__ WideWorkspace<Q, W> ws;
-    __shared__ uint32_t item_sh;
+    __shared__ uint32_t item_sh, item_hi_sh;
     const uint32_t start = a.bin_ranges[2 * a.variant];
     const uint32_t count = a.bin_ranges[2 * a.variant + 1];
     const uint32_t cf = a.chunk_factor ? a.chunk_factor : 1u;
+    uint32_t *heads = a.queue_heads + a.variant * (uint32_t)(MYERS_QUEUE_PARTS * MYERS_QUEUE_STRIDE);
+    uint32_t part = 0;
     zero_row_init_wide<Q, W>(ws);
     uint32_t staged = 0xFFFFFFFFu;
     for (;;) {
         __syncthreads();
-        if (threadIdx.x == 0) item_sh = atomicAdd(&a.queue_heads[a.variant], cf);
+        if (threadIdx.x == 0) {
+            uint32_t lo = 0, hi = 0;
+            if (!pull_work(heads, count, cf, part, lo, hi)) lo = hi = 0xFFFFFFFFu;
+            item_sh = lo;
+            item_hi_sh = hi;
+        }
         __syncthreads();
         const uint32_t lo = __builtin_amdgcn_readfirstlane(item_sh);
-        if (lo >= count) break;
-        verify_wide<Q, W>(a, start, count, lo, min(lo + cf, count), 1u, ws, staged);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane(item_hi_sh);
+        if (lo == 0xFFFFFFFFu) break;
+        verify_wide<Q, W>(a, start, count, lo, hi, 1u, ws, staged);
     }
 }
 
 // ---------------------------------------------------------------- binning
 
-__global__ __launch_bounds__(64) void bin_reset_kernel(uint32_t *counts, uint32_t *err, uint32_t nv) {
-    for (uint32_t i = threadIdx.x; i < nv; i += blockDim.x) counts[i] = 0;
+__global__ __launch_bounds__(64) void bin_reset_kernel(uint32_t *err) {
     if (threadIdx.x == 0) *err = 0;
 }
 
@@ -776,15 +822,30 @@ __device__ __forceinline__ uint32_t variant_of(const uint2 *qinfo, uint32_t num_
     return w <= max_words ? (uint32_t)bin_of_words[w] : kTooLong;
 }
 
+// Binning is a stable partition of the candidates by variant with no global
+// atomics: block b of the count and scatter kernels owns the contiguous span
+// [b*span, (b+1)*span); bin_count writes each block's per-variant counts
+// (variant-major), bin_scan turns them into each block's first slot per
+// variant and the variants' ranges, and bin_scatter writes every candidate's
+// id at its slot in input order.  A read's candidates, adjacent in the
+// input, stay adjacent in their bin.
+
+__device__ __forceinline__ uint32_t bin_span(uint32_t n) {
+    const uint32_t span = (n + gridDim.x - 1u) / gridDim.x;
+    return (span + 255u) & ~255u;
+}
+
 __global__ __launch_bounds__(256) void bin_count_kernel(const uint2 *qinfo, uint32_t num_queries,
                                                         const CandRec *cands,
                                                         uint32_t n, const uint8_t *bin_of_words,
                                                         uint32_t max_words, uint32_t nv,
-                                                        uint32_t *counts, uint32_t *err) {
+                                                        uint32_t *block_counts, uint32_t *err) {
     __shared__ uint32_t hist[MYERS_MAX_VARIANTS];
     for (uint32_t i = threadIdx.x; i < nv; i += blockDim.x) hist[i] = 0;
     __syncthreads();
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t span = bin_span(n);
+    const uint32_t lo = blockIdx.x * span, hi = min(n, lo + span);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
         const uint32_t v = variant_of(qinfo, num_queries, cands, i, bin_of_words, max_words);
         if (v < nv) {
             atomicAdd(&hist[v], 1u);
@@ -793,55 +854,101 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const uint2 *qinfo, uint
         }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < nv; i += blockDim.x)
-        if (hist[i]) atomicAdd(&counts[i], hist[i]);
+    for (uint32_t v = threadIdx.x; v < nv; v += blockDim.x) block_counts[v * gridDim.x + blockIdx.x] = hist[v];
 }
 
-__global__ __launch_bounds__(64) void bin_scan_kernel(const uint32_t *counts, uint32_t nv,
-                                                      uint32_t *ranges, uint32_t *cursors,
-                                                      uint32_t *queue_heads) {
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (uint32_t v = 0; v < nv; v++) {
-            ranges[2 * v] = acc;
-            ranges[2 * v + 1] = counts[v];
-            acc += counts[v];
+// One block of 256 threads: for every variant, an exclusive scan over the
+// count blocks (16 consecutive blocks per thread, then a scan of the 256
+// thread sums), turning block_counts into first slots in place.
+__global__ __launch_bounds__(256) void bin_scan_kernel(uint32_t *block_counts, uint32_t nblocks, uint32_t nv,
+                                                       uint32_t *ranges, uint32_t *queue_heads) {
+    __shared__ uint32_t part[256];
+    __shared__ uint32_t base_sh;
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nblocks + 255u) / 256u;
+    if (t == 0) base_sh = 0;
+    for (uint32_t v = 0; v < nv; v++) {
+        uint32_t *row = block_counts + v * nblocks;
+        uint32_t sum = 0;
+        for (uint32_t k = 0; k < per; k++) {
+            const uint32_t b = t * per + k;
+            if (b < nblocks) sum += row[b];
+        }
+        __syncthreads();
+        part[t] = sum;
+        __syncthreads();
+        // inclusive scan of the 256 thread sums (Hillis-Steele)
+        for (uint32_t o = 1; o < 256u; o <<= 1) {
+            const uint32_t x = t >= o ? part[t - o] : 0u;
+            __syncthreads();
+            part[t] += x;
+            __syncthreads();
+        }
+        const uint32_t base = base_sh;
+        uint32_t run = base + part[t] - sum;           // exclusive prefix of this thread's blocks
+        for (uint32_t k = 0; k < per; k++) {
+            const uint32_t b = t * per + k;
+            if (b < nblocks) {
+                const uint32_t c = row[b];
+                row[b] = run;
+                run += c;
+            }
+        }
+        __syncthreads();
+        if (t == 0) {
+            ranges[2 * v] = base;
+            ranges[2 * v + 1] = part[255];
+            base_sh = base + part[255];
         }
     }
-    for (uint32_t v = threadIdx.x; v < nv; v += blockDim.x) {
-        cursors[v] = 0;
-        queue_heads[v] = 0;
-    }
+    for (uint32_t i = t; i < nv * (uint32_t)MYERS_QUEUE_PARTS; i += blockDim.x)
+        queue_heads[i * (uint32_t)MYERS_QUEUE_STRIDE] = 0;
 }
 
-// Scatter candidate ids into their variant's slots.  Lanes of one wave that
-// share a variant take consecutive slots in lane order (one atomic per
-// variant per wave), so the candidates of one query stay adjacent.
+// Write each candidate's id at its slot: per 256-candidate round, ranks within
+// a wave come from ballots, the waves' counts are added in wave order in LDS,
+// and the block's running first slot per variant advances by the round's
+// counts.
 __global__ __launch_bounds__(256) void bin_scatter_kernel(const uint2 *qinfo, uint32_t num_queries,
                                                           const CandRec *cands,
                                                           uint32_t n, const uint8_t *bin_of_words,
                                                           uint32_t max_words, uint32_t nv,
-                                                          const uint32_t *ranges, uint32_t *cursors,
-                                                          uint32_t *order, uint2 *results) {
+                                                          const uint32_t *block_first, uint32_t *order,
+                                                          uint2 *results) {
+    __shared__ uint32_t next[MYERS_MAX_VARIANTS];
+    __shared__ uint32_t wave_cnt[4][MYERS_MAX_VARIANTS];
     const uint32_t lane = __lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t rounds = (n + stride - 1) / stride;
-    for (uint32_t r = 0; r < rounds; r++) {
-        const uint32_t i = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
-        const uint32_t v = i < n ? variant_of(qinfo, num_queries, cands, i, bin_of_words, max_words) : kTooLong;
-        if (i < n && v >= nv) results[i] = make_uint2(0u, kSkipScore);
+    for (uint32_t v = threadIdx.x; v < nv; v += blockDim.x) next[v] = block_first[v * gridDim.x + blockIdx.x];
+    const uint32_t span = bin_span(n);
+    const uint32_t lo = blockIdx.x * span, hi = min(n, lo + span);
+    for (uint32_t r = lo; r < hi; r += 256u) {
+        for (uint32_t i = threadIdx.x; i < 4u * nv; i += blockDim.x) wave_cnt[i / nv][i % nv] = 0;
+        __syncthreads();
+        const uint32_t i = r + threadIdx.x;
+        const uint32_t v = i < hi ? variant_of(qinfo, num_queries, cands, i, bin_of_words, max_words) : kTooLong;
+        if (i < hi && v >= nv) results[i] = make_uint2(0u, kSkipScore);
+        uint32_t rank = 0;
         uint64_t pending = __ballot(v < nv);
         while (pending) {
             const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
             const uint32_t vv = (uint32_t)__shfl((int)v, (int)leader);
             const uint64_t mine = __ballot(v == vv);
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&cursors[vv], (uint32_t)__builtin_popcountll(mine));
-            base = (uint32_t)__shfl((int)base, (int)leader);
-            if (v == vv) order[ranges[2 * vv] + base + (uint32_t)__builtin_popcountll(mine & below)] = i;
+            if (v == vv) rank = (uint32_t)__builtin_popcountll(mine & below);
+            if (lane == leader) wave_cnt[wave][vv] = (uint32_t)__builtin_popcountll(mine);
             pending &= ~mine;
         }
+        __syncthreads();
+        if (v < nv) {
+            uint32_t slot = next[v] + rank;
+            for (uint32_t w = 0; w < wave; w++) slot += wave_cnt[w][v];
+            order[slot] = i;
+        }
+        __syncthreads();
+        for (uint32_t vv = threadIdx.x; vv < nv; vv += blockDim.x)
+            next[vv] += wave_cnt[0][vv] + wave_cnt[1][vv] + wave_cnt[2][vv] + wave_cnt[3][vv];
+        __syncthreads();
     }
 }
 
@@ -943,22 +1050,22 @@ extern "C" int myers_launch_verify(uint32_t v, const myers_verify_args_t *args, 
 extern "C" int myers_launch_binning(const uint32_t *qinfo, uint32_t num_queries, const void *cands,
                                     uint32_t num_cands,
                                     const uint8_t *bin_of_words, uint32_t max_words,
-                                    uint32_t num_variants, uint32_t *counts, uint32_t *cursors,
-                                    uint32_t *ranges, uint32_t *order, uint32_t *err,
-                                    void *results, uint32_t *queue_heads, hipStream_t stream) {
+                                    uint32_t num_variants, uint32_t *block_counts, uint32_t *ranges,
+                                    uint32_t *order, uint32_t *err, void *results, uint32_t *queue_heads,
+                                    hipStream_t stream) {
     if (num_variants == 0 || num_variants > MYERS_MAX_VARIANTS) return -1;
     const uint2 *qi = reinterpret_cast<const uint2 *>(qinfo);
     const CandRec *cr = static_cast<const CandRec *>(cands);
-    uint32_t grid = (num_cands + 255u) / 256u;
-    if (grid > 4096u) grid = 4096u;
+    uint32_t grid = (num_cands + 1023u) / 1024u;           // about 4 rounds of 256 per block
+    if (grid > (uint32_t)MYERS_BIN_BLOCKS) grid = (uint32_t)MYERS_BIN_BLOCKS;
     if (grid == 0u) grid = 1u;
-    hipLaunchKernelGGL(bin_reset_kernel, dim3(1), dim3(64), 0, stream, counts, err, num_variants);
+    hipLaunchKernelGGL(bin_reset_kernel, dim3(1), dim3(64), 0, stream, err);
     hipLaunchKernelGGL(bin_count_kernel, dim3(grid), dim3(256), 0, stream, qi, num_queries, cr, num_cands,
-                       bin_of_words, max_words, num_variants, counts, err);
-    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(64), 0, stream, counts, num_variants, ranges,
-                       cursors, queue_heads);
+                       bin_of_words, max_words, num_variants, block_counts, err);
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(256), 0, stream, block_counts, grid, num_variants, ranges,
+                       queue_heads);
     hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid), dim3(256), 0, stream, qi, num_queries, cr, num_cands,
-                       bin_of_words, max_words, num_variants, ranges, cursors, order,
+                       bin_of_words, max_words, num_variants, block_counts, order,
                        static_cast<uint2 *>(results));
     return (int)hipGetLastError();
 }

@@ -468,7 +468,8 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
 // dispatched round-robin over the 8 XCDs) and moves on to the other parts
 // when its own is drained, so the atomics of one part stay within one XCD
 // until the tail.  Each pull takes a guided share of what is left in the
-// part (at least min_take items).  Thread 0 only; *part counts the parts
+// part, an eighth of an even split over the part's workgroups (at least
+// min_take items), so long-running items still balance at the tail.  Thread 0 only; *part counts the parts
 // this workgroup has found drained.  Returns false when all are.
 __device__ __forceinline__ bool pull_work(uint32_t *heads, uint32_t n, uint32_t min_take, uint32_t &part,
                                           uint32_t &lo, uint32_t &hi) {
@@ -481,7 +482,7 @@ __device__ __forceinline__ bool pull_work(uint32_t *heads, uint32_t n, uint32_t 
         uint32_t *head = heads + x * (uint32_t)MYERS_QUEUE_STRIDE;
         const uint32_t seen = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint32_t left = (p_hi - p_lo) > seen ? (p_hi - p_lo) - seen : 0u;
-        const uint32_t take = max(min_take, left / (2u * workers));
+        const uint32_t take = max(min_take, left / (8u * workers));
         const uint32_t got = atomicAdd(head, take);
         if (got < p_hi - p_lo) {
             lo = p_lo + got;

@@ -482,6 +482,10 @@ __device__ __forceinline__ bool pull_work(uint32_t *heads, uint32_t n, uint32_t 
         uint32_t *head = heads + x * (uint32_t)MYERS_QUEUE_STRIDE;
         const uint32_t seen = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint32_t left = (p_hi - p_lo) > seen ? (p_hi - p_lo) - seen : 0u;
+        if (left == 0u) {                               // drained: no atomic needed
+            part++;
+            continue;
+        }
         const uint32_t take = max(min_take, left / (8u * workers));
         const uint32_t got = atomicAdd(head, take);
         if (got < p_hi - p_lo) {
@@ -495,10 +499,12 @@ __device__ __forceinline__ bool pull_work(uint32_t *heads, uint32_t n, uint32_t 
 }
 
 // Waves per SIMD the register allocator must leave room for (1 = no limit).
-// The wide one-lane kernels fit 3 (<= 168 VGPRs) or 4 (<= 128) without
-// spilling; more resident waves keep the double-rate logic ops issuing.
+// The wide one-lane kernels fit 3 (<= 168 VGPRs) or 4 (<= 128); more
+// resident waves keep the double-rate logic ops issuing.  The cut-off
+// kernels for W >= 28 then spill a few per-chunk values, outside the column
+// loop.
 template <int L, int W, bool kCut, bool kPersistent>
-constexpr int kWavesPerEU = (L != 1 || kCut) ? 1 : (W >= 28 ? 3 : ((W >= 20 && !kPersistent) ? 4 : 1));
+constexpr int kWavesPerEU = L != 1 ? 1 : (W >= 28 ? 3 : ((W >= 20 && !kPersistent && !kCut) ? 4 : 1));
 
 // Plain launch: a fixed grid strides over the chunks of one slot range.
 template <int L, int W, bool kCut>
@@ -536,6 +542,7 @@ void verify_persistent_kernel(myers_verify_args_t a) {
     const uint32_t cf = a.chunk_factor ? a.chunk_factor : 1u;
     uint32_t *heads = a.queue_heads + a.variant * (uint32_t)(MYERS_QUEUE_PARTS * MYERS_QUEUE_STRIDE);
     uint32_t part = 0;
+    if (blockIdx.x >= nchunks) return;      // more workgroups than chunks: the others cover them
     zero_row_init<L, W>(ws);
     for (;;) {
         __syncthreads();
@@ -788,6 +795,7 @@ __global__ __launch_bounds__(64 * Q) void verify_wide_persistent_kernel(myers_ve
     const uint32_t cf = a.chunk_factor ? a.chunk_factor : 1u;
     uint32_t *heads = a.queue_heads + a.variant * (uint32_t)(MYERS_QUEUE_PARTS * MYERS_QUEUE_STRIDE);
     uint32_t part = 0;
+    if (blockIdx.x >= count) return;        // more workgroups than candidates: the others cover them
     zero_row_init_wide<Q, W>(ws);
     uint32_t staged = 0xFFFFFFFFu;
     for (;;) {
@@ -826,8 +834,9 @@ __device__ __forceinline__ uint32_t variant_of(const uint2 *qinfo, uint32_t num_
 // Binning is a stable partition of the candidates by variant with no global
 // atomics: block b of the count and scatter kernels owns the contiguous span
 // [b*span, (b+1)*span); bin_count writes each block's per-variant counts
-// (variant-major), bin_scan turns them into each block's first slot per
-// variant and the variants' ranges, and bin_scatter writes every candidate's
+// (variant-major), bin_scan (one block per variant) turns them into each
+// block's first slot within the variant, bin_base lays the variants' ranges
+// end to end, and bin_scatter writes every candidate's
 // id at its slot in input order.  A read's candidates, adjacent in the
 // input, stay adjacent in their bin.
 
@@ -858,51 +867,51 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const uint2 *qinfo, uint
     for (uint32_t v = threadIdx.x; v < nv; v += blockDim.x) block_counts[v * gridDim.x + blockIdx.x] = hist[v];
 }
 
-// One block of 256 threads: for every variant, an exclusive scan over the
-// count blocks (16 consecutive blocks per thread, then a scan of the 256
-// thread sums), turning block_counts into first slots in place.
-__global__ __launch_bounds__(256) void bin_scan_kernel(uint32_t *block_counts, uint32_t nblocks, uint32_t nv,
-                                                       uint32_t *ranges, uint32_t *queue_heads) {
+// One block of 256 threads per variant: an exclusive scan of the variant's
+// per-block counts (16 consecutive blocks per thread, then a scan of the 256
+// thread sums), in place, giving each count block's first slot relative to
+// the variant's range; the variant's total goes to ranges[2v+1].
+__global__ __launch_bounds__(256) void bin_scan_kernel(uint32_t *block_counts, uint32_t nblocks,
+                                                       uint32_t *ranges) {
     __shared__ uint32_t part[256];
-    __shared__ uint32_t base_sh;
-    const uint32_t t = threadIdx.x;
+    const uint32_t t = threadIdx.x, v = blockIdx.x;
     const uint32_t per = (nblocks + 255u) / 256u;
-    if (t == 0) base_sh = 0;
-    for (uint32_t v = 0; v < nv; v++) {
-        uint32_t *row = block_counts + v * nblocks;
-        uint32_t sum = 0;
-        for (uint32_t k = 0; k < per; k++) {
-            const uint32_t b = t * per + k;
-            if (b < nblocks) sum += row[b];
-        }
+    uint32_t *row = block_counts + v * nblocks;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < per; k++) {
+        const uint32_t b = t * per + k;
+        if (b < nblocks) sum += row[b];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t o = 1; o < 256u; o <<= 1) {       // inclusive scan (Hillis-Steele)
+        const uint32_t x = t >= o ? part[t - o] : 0u;
         __syncthreads();
-        part[t] = sum;
+        part[t] += x;
         __syncthreads();
-        // inclusive scan of the 256 thread sums (Hillis-Steele)
-        for (uint32_t o = 1; o < 256u; o <<= 1) {
-            const uint32_t x = t >= o ? part[t - o] : 0u;
-            __syncthreads();
-            part[t] += x;
-            __syncthreads();
-        }
-        const uint32_t base = base_sh;
-        uint32_t run = base + part[t] - sum;           // exclusive prefix of this thread's blocks
-        for (uint32_t k = 0; k < per; k++) {
-            const uint32_t b = t * per + k;
-            if (b < nblocks) {
-                const uint32_t c = row[b];
-                row[b] = run;
-                run += c;
-            }
-        }
-        __syncthreads();
-        if (t == 0) {
-            ranges[2 * v] = base;
-            ranges[2 * v + 1] = part[255];
-            base_sh = base + part[255];
+    }
+    uint32_t run = part[t] - sum;
+    for (uint32_t k = 0; k < per; k++) {
+        const uint32_t b = t * per + k;
+        if (b < nblocks) {
+            const uint32_t c = row[b];
+            row[b] = run;
+            run += c;
         }
     }
-    for (uint32_t i = t; i < nv * (uint32_t)MYERS_QUEUE_PARTS; i += blockDim.x)
+    if (t == 255u) ranges[2 * v + 1] = part[255];
+}
+
+// Variant ranges from the totals, and the persistent queues' part heads.
+__global__ __launch_bounds__(64) void bin_base_kernel(uint32_t nv, uint32_t *ranges, uint32_t *queue_heads) {
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t v = 0; v < nv; v++) {
+            ranges[2 * v] = acc;
+            acc += ranges[2 * v + 1];
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < nv * (uint32_t)MYERS_QUEUE_PARTS; i += blockDim.x)
         queue_heads[i * (uint32_t)MYERS_QUEUE_STRIDE] = 0;
 }
 
@@ -914,14 +923,15 @@ __global__ __launch_bounds__(256) void bin_scatter_kernel(const uint2 *qinfo, ui
                                                           const CandRec *cands,
                                                           uint32_t n, const uint8_t *bin_of_words,
                                                           uint32_t max_words, uint32_t nv,
-                                                          const uint32_t *block_first, uint32_t *order,
-                                                          uint2 *results) {
+                                                          const uint32_t *block_first, const uint32_t *ranges,
+                                                          uint32_t *order, uint2 *results) {
     __shared__ uint32_t next[MYERS_MAX_VARIANTS];
     __shared__ uint32_t wave_cnt[4][MYERS_MAX_VARIANTS];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
-    for (uint32_t v = threadIdx.x; v < nv; v += blockDim.x) next[v] = block_first[v * gridDim.x + blockIdx.x];
+    for (uint32_t v = threadIdx.x; v < nv; v += blockDim.x)
+        next[v] = ranges[2 * v] + block_first[v * gridDim.x + blockIdx.x];
     const uint32_t span = bin_span(n);
     const uint32_t lo = blockIdx.x * span, hi = min(n, lo + span);
     for (uint32_t r = lo; r < hi; r += 256u) {
@@ -1063,10 +1073,10 @@ extern "C" int myers_launch_binning(const uint32_t *qinfo, uint32_t num_queries,
     hipLaunchKernelGGL(bin_reset_kernel, dim3(1), dim3(64), 0, stream, err);
     hipLaunchKernelGGL(bin_count_kernel, dim3(grid), dim3(256), 0, stream, qi, num_queries, cr, num_cands,
                        bin_of_words, max_words, num_variants, block_counts, err);
-    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(256), 0, stream, block_counts, grid, num_variants, ranges,
-                       queue_heads);
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(num_variants), dim3(256), 0, stream, block_counts, grid, ranges);
+    hipLaunchKernelGGL(bin_base_kernel, dim3(1), dim3(64), 0, stream, num_variants, ranges, queue_heads);
     hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid), dim3(256), 0, stream, qi, num_queries, cr, num_cands,
-                       bin_of_words, max_words, num_variants, block_counts, order,
+                       bin_of_words, max_words, num_variants, block_counts, ranges, order,
                        static_cast<uint2 *>(results));
     return (int)hipGetLastError();
 }

@@ -211,15 +211,15 @@ def serving_pool(ref, ref_size, seed, n_requests):
     return reqs
 
 
-def measure_serving(m, ref, ref_size, loads, seconds, batch_cells=0, seed=5):
+def measure_serving(m, ref, ref_size, loads, seconds, batch_cells=0, engine=0, seed=5):
     pool = serving_pool(ref, ref_size, seed, 192)
     cells = [int((r[1][r[2]["query"], 1].astype(np.int64) * window_len(r[1][r[2]["query"], 1])).sum()) for r in pool]
     cands = [len(r[2]) for r in pool]
     alone = [m.verify(*r, DISTANCE) for r in pool[:24]]          # reference results for the mix check
     out = {"pool_requests": len(pool), "mean_reads_per_request": float(np.mean([len(r[1]) for r in pool])),
            "mean_candidates_per_request": float(np.mean(cands)), "buffers": m.num_buffers,
-           "max_batch_cells": batch_cells or "library default (2.5e11)", "levels": []}
-    with M.Server(m, distance=DISTANCE, max_batch_cells=batch_cells) as sv:
+           "max_batch_cells": batch_cells or "library default (2.5e11)", "engine": engine, "levels": []}
+    with M.Server(m, distance=DISTANCE, flags=engine, max_batch_cells=batch_cells) as sv:
         # saturated throughput: every pool request arrives at once
         made = [M.Server.request(*r) for r in pool]
         lat, wall = sv.run_open_loop([x[0] for x in made], np.zeros(len(made), dtype=np.uint64))
@@ -267,8 +267,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--serve-loads", default="0.7", help="serving load levels (fractions of saturation); '' = skip")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
-    ap.add_argument("--serve-buffers", type=int, default=6)
+    ap.add_argument("--serve-buffers", type=int, default=3)
     ap.add_argument("--serve-batch-cells", type=float, default=0, help="serving batch cap on cells (0 = library default)")
+    ap.add_argument("--serve-engine", default="auto", choices=["auto", "plain", "persistent"])
     ap.add_argument("--variant", default=None, help="force a kernel variant for every read length, e.g. L2xW16")
     ap.add_argument("--threshold", type=float, default=0.2,
                     help="also time threshold mode at this error rate (negative = skip)")
@@ -367,7 +368,7 @@ def main():
     if serve_loads:
         log("serving ...")
         serving = measure_serving(m, ref, cfg["ref_size"], serve_loads, args.serve_seconds,
-                                  int(args.serve_batch_cells))
+                                  int(args.serve_batch_cells), engines[args.serve_engine])
 
     out = {
         "metric": "candidates/s & GCUPS on 1 MI355X (bit-exact); p99 latency under serving load",

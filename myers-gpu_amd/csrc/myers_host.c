@@ -97,7 +97,7 @@ typedef struct myers_buffer {
     hipEvent_t   ev_fork, ev_join[SIDE_STREAMS];
     /* steady-state graph of the last device-resident batch shape */
     hipGraphExec_t graph_exec;
-    uint64_t     graph_key[11];
+    uint64_t     graph_key[12];
     int          graph_valid;
     int          graph_binned;
     uint32_t     graph_launches;
@@ -510,17 +510,22 @@ typedef struct {
     uint32_t num_present;
     uint32_t single;       /* the variant when num_present == 1 */
     int      too_long;
-    uint64_t cells_per_cand_sum_unused;
+    int      sized;        /* reads[] below is filled in */
+    uint32_t reads[MYERS_MAX_VARIANTS];   /* reads per variant */
 } bin_scan_t;
 
 static void scan_qinfo(const myers_ctx_t *ctx, const qryInfo_t *qinfo, uint32_t nq, bin_scan_t *s)
 {
     const uint32_t maxw = max_supported_words(ctx);
     memset(s, 0, sizeof(*s));
+    s->sized = 1;
     for (uint32_t q = 0; q < nq; q++) {
-        uint32_t w = (qinfo[q].size + 31u) >> 5;
+        const uint32_t m = qinfo[q].size;
+        uint32_t w = (m + 31u) >> 5;
         if (w > maxw) { s->too_long = 1; continue; }
-        s->present |= 1ull << ctx->bin_of_words[w];
+        const uint32_t v = ctx->bin_of_words[w];
+        s->present |= 1ull << v;
+        s->reads[v]++;
     }
     for (uint32_t v = 0; v < ctx->num_variants; v++)
         if (s->present & (1ull << v)) { s->num_present++; s->single = v; }
@@ -611,6 +616,38 @@ static uint32_t present_by_cost(const myers_ctx_t *ctx, uint64_t present, uint32
     return n;
 }
 
+/* Long-read bins of a persistent batch that go into one launch
+ * (myers_launch_long): bins of 4..64 lanes per candidate, longest reads
+ * first, when their chunks (estimated from the reads per bin and the batch's
+ * candidates per read) fit a few rounds of one wave per SIMD.  Such bins are
+ * bound by each candidate's chain of column steps; larger long-read batches
+ * are throughput work and keep one launch per bin.  Returns the bins taken
+ * (0: none). */
+#define LONG_ROUNDS 4u
+static uint32_t long_plan(const myers_ctx_t *ctx, const bin_scan_t *scan, const batch_t *bt,
+                          const uint32_t *order, uint32_t n, myers_long_plan_t *lp)
+{
+    memset(lp, 0, sizeof(*lp));
+    if (!scan->sized || bt->num_queries == 0) return 0;
+    const double cpr = (double)bt->num_cands / (double)bt->num_queries;
+    double items = 0.0;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t v = order[i];
+        uint32_t u4 = 0;
+        if (!myers_variant_long_ok(v, &u4) || lp->n == MYERS_LONG_MAX) continue;
+        lp->variant[lp->n] = v;
+        lp->key[lp->n] = (ctx->variants[v].lanes << 8) | ctx->variants[v].words;
+        lp->n++;
+        if (u4 > lp->ws_u4) lp->ws_u4 = u4;
+        items += (double)scan->reads[v] * cpr / (double)ctx->variants[v].slots;
+    }
+    if (lp->n == 0 || items > (double)(LONG_ROUNDS * MYERS_LONG_WAVES) * (double)ctx->num_cus) {
+        lp->n = 0;
+        return 0;
+    }
+    return lp->n;
+}
+
 /* Enqueue the whole on-device pipeline of one batch on b->stream (no timing
  * events, so that it can be captured into a graph). */
 static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan_t *scan)
@@ -662,6 +699,7 @@ static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan
     a.queue_heads = heads;
     uint32_t order[MYERS_MAX_VARIANTS];
     const uint32_t n = present_by_cost(ctx, scan->present, order);
+    uint32_t launches = n;
     if (engine == MYERS_ENGINE_PLAIN) {
         /* one launch per length bin, in turn */
         for (uint32_t i = 0; i < n; i++) {
@@ -671,16 +709,33 @@ static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan
         }
     } else {
         /* persistent: every bin's resident grid drains its queue; the bins run
-         * on forked streams so that one bin's tail overlaps the next bin */
-        const uint32_t width = n < SIDE_STREAMS ? n : SIDE_STREAMS;
+         * on forked streams so that one bin's tail overlaps the next bin.  The
+         * long-read bins of a latency-bound batch share one launch on the
+         * first side stream, the others go round-robin on the rest. */
+        myers_long_plan_t lp;
+        const uint32_t nlong = long_plan(ctx, scan, bt, order, n, &lp);
+        uint32_t rest[MYERS_MAX_VARIANTS], nrest = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            int in_long = 0;
+            for (uint32_t j = 0; j < nlong; j++) in_long |= lp.variant[j] == order[i];
+            if (!in_long) rest[nrest++] = order[i];
+        }
+        launches = nrest + (nlong ? 1u : 0u);
+        const uint32_t width = launches < SIDE_STREAMS ? launches : SIDE_STREAMS;
+        const uint32_t first = nlong ? 1u : 0u;
         HIP_TRY(hipEventRecord(b->ev_fork, b->stream));
         for (uint32_t j = 0; j < width; j++) HIP_TRY(hipStreamWaitEvent(b->side[j], b->ev_fork, 0));
-        for (uint32_t i = 0; i < n; i++) {
-            const uint32_t v = order[i];
+        if (nlong) {
+            rc = myers_launch_long(&a, &lp, ctx->num_cus, b->side[0]);
+            if (rc) return hip_fail((hipError_t)rc, "long-read launch", __FILE__, __LINE__);
+        }
+        for (uint32_t i = 0; i < nrest; i++) {
+            const uint32_t v = rest[i];
             a.variant = v;
             a.chunk_factor = ctx->chunk_factor[v];
             uint64_t grid = (uint64_t)ctx->num_cus * (uint64_t)ctx->occupancy_p[v];
-            rc = myers_launch_verify(v, &a, (uint32_t)grid, 1, b->side[i % width]);
+            const uint32_t side = width > first ? first + i % (width - first) : 0u;
+            rc = myers_launch_verify(v, &a, (uint32_t)grid, 1, b->side[side]);
             if (rc) return hip_fail((hipError_t)rc, "persistent launch", __FILE__, __LINE__);
         }
         for (uint32_t j = 0; j < width; j++) {
@@ -688,7 +743,7 @@ static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan
             HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_join[j], 0));
         }
     }
-    b->stats.launches = n;
+    b->stats.launches = launches;
     b->stats.variantsUsed = n;
     return MYERS_SUCCESS;
 }
@@ -722,12 +777,15 @@ static int enqueue_verify(myers_buffer_t *b, const batch_t *bt, const qryInfo_t 
     uint32_t dist_bits, thr_bits;
     memcpy(&dist_bits, &bt->distance, sizeof(dist_bits));
     memcpy(&thr_bits, &bt->threshold, sizeof(thr_bits));
-    const uint64_t key[11] = {(uint64_t)(uintptr_t)bt->d_peq, (uint64_t)(uintptr_t)bt->d_qinfo,
+    uint32_t order[MYERS_MAX_VARIANTS];
+    myers_long_plan_t lp;
+    const uint64_t plan_key = long_plan(b->ctx, &scan, bt, order, present_by_cost(b->ctx, scan.present, order), &lp);
+    const uint64_t key[12] = {(uint64_t)(uintptr_t)bt->d_peq, (uint64_t)(uintptr_t)bt->d_qinfo,
                               (uint64_t)(uintptr_t)bt->d_cands, (uint64_t)(uintptr_t)bt->d_results,
                               ((uint64_t)bt->num_entries << 32) | bt->num_queries,
                               ((uint64_t)bt->num_cands << 32) | bt->flags,
                               (uint64_t)dist_bits, scan.present,
-                              (uint64_t)scan.too_long, b->ctx->tune_gen, (uint64_t)thr_bits};
+                              (uint64_t)scan.too_long, b->ctx->tune_gen, (uint64_t)thr_bits, plan_key};
     if (!b->graph_valid || memcmp(key, b->graph_key, sizeof(key)) != 0) {
         if (b->graph_exec) {
             hipGraphExecDestroy(b->graph_exec);

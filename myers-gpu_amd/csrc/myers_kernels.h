@@ -69,6 +69,25 @@ int myers_launch_verify(uint32_t v, const myers_verify_args_t *args, uint32_t gr
 /* Resident workgroups per CU of the persistent kernel of variant v. */
 int myers_variant_occupancy_persistent(uint32_t v, int *wg_per_cu);
 
+/* Long-read bins in one launch.  Bins of one-wave variants with 4..64 lanes
+ * per candidate (myers_variant_long_ok) can run together in workgroups of
+ * MYERS_LONG_WAVES independent waves; each wave pulls the next chunk from
+ * one counter (queue_heads + MYERS_LONG_HEAD, reset by the binning) that
+ * runs over the plan's bins in order.  Needs bin_ranges and queue_heads. */
+enum { MYERS_LONG_MAX = 16, MYERS_LONG_WAVES = 4 };
+#define MYERS_LONG_HEAD ((MYERS_MAX_VARIANTS - 1) * MYERS_QUEUE_PARTS * MYERS_QUEUE_STRIDE)
+typedef struct {
+    uint32_t n;                          /* bins, in pull order */
+    uint32_t ws_u4;                      /* LDS per wave in 16-byte units (max over the bins) */
+    uint32_t variant[MYERS_LONG_MAX];    /* variant ids (rows of bin_ranges) */
+    uint32_t key[MYERS_LONG_MAX];        /* lanes << 8 | words of each */
+} myers_long_plan_t;
+/* 1 if variant v can run in the long-read launch, and its LDS per wave. */
+int myers_variant_long_ok(uint32_t v, uint32_t *ws_u4);
+/* grid = workgroups of MYERS_LONG_WAVES waves. */
+int myers_launch_long(const myers_verify_args_t *args, const myers_long_plan_t *plan, uint32_t grid,
+                      hipStream_t stream);
+
 /* Device-side length binning: a stable partition of the candidates by
  * variant.  bin_of_words[w] (w = ceil(m/32), 0..max_words) gives the variant
  * of a query of w words.  block_counts holds MYERS_BIN_WORDS u32 of

@@ -210,14 +210,37 @@ struct Workspace {
     uint2 qtab[Shape<L, W>::kQSlots];
 };
 
-template <int L, int W>
+// LDS hand-over between the lanes of one wave when a workgroup holds several
+// independent waves: a wave's LDS accesses execute in order, so it is enough
+// to wait for them and keep the compiler from moving accesses across.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Workgroups of kWaves independent waves (1: the workgroup is the wave).
+template <int kWaves>
+__device__ __forceinline__ void chunk_sync() {
+    if constexpr (kWaves == 1) {
+        __syncthreads();
+    } else {
+        wave_lds_sync();
+    }
+}
+
+template <int kWaves>
+__device__ __forceinline__ uint32_t wave_lane() {
+    return kWaves == 1 ? threadIdx.x : (threadIdx.x & 63u);
+}
+
+template <int L, int W, int kWaves = 1>
 __device__ __forceinline__ void zero_row_init(Workspace<L, W> &ws) {
     using S = Shape<L, W>;
     if constexpr (S::kRows > 5) {
         // the all-zero row read by lanes before their first column
         uint32_t *l32 = reinterpret_cast<uint32_t *>(ws.lds);
         constexpr uint32_t kPerGroup = (uint32_t)(S::kQSlots * L);    // (slot, lane) pairs
-        for (uint32_t i = threadIdx.x; i < (uint32_t)S::kGroups * kPerGroup * 4u; i += 64u) {
+        for (uint32_t i = wave_lane<kWaves>(); i < (uint32_t)S::kGroups * kPerGroup * 4u; i += 64u) {
             const uint32_t u4 = i >> 2, g = u4 / kPerGroup, r = u4 % kPerGroup;
             const uint32_t slot = r / (uint32_t)L, gl = r % (uint32_t)L;
             l32[(g * (uint32_t)S::kGroupU4 + slot * (uint32_t)S::kSlotU4 + kZeroRow * (uint32_t)L + gl) * 4u +
@@ -229,14 +252,14 @@ __device__ __forceinline__ void zero_row_init(Workspace<L, W> &ws) {
 // Verify chunks chunk_lo, chunk_lo + stride, ... < chunk_hi of the slot range
 // [start, start + count); a chunk is the kSlots candidates of one wave.
 // kCut (L = 1, W % 4 == 0): threshold mode with the Ukkonen cut-off.
-template <int L, int W, bool kCut>
+template <int L, int W, bool kCut, int kWaves = 1>
 __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint32_t start, uint32_t count,
                                               uint32_t chunk_lo, uint32_t chunk_hi, uint32_t stride,
                                               Workspace<L, W> &ws) {
     using S = Shape<L, W>;
     uint4 *lds = ws.lds;
     uint2 *qtab = ws.qtab;
-    const uint32_t lane = threadIdx.x;
+    const uint32_t lane = wave_lane<kWaves>();
     const uint32_t gl = lane % L;                      // lane within the candidate's group
     const uint32_t *__restrict__ ref = a.ref;
     const int32_t last_word = (int32_t)(a.ref_words - 1);
@@ -296,11 +319,11 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
         for (uint32_t base_slot = 0; base_slot < nqslots; base_slot += S::kQSlots) {
             const bool in_pass = qslot >= base_slot && qslot < base_slot + S::kQSlots;
             const uint32_t my_slot = in_pass ? qslot - base_slot : 0u;
-            __syncthreads();                             // previous readers of qtab / lds are done
+            chunk_sync<kWaves>();                        // previous readers of qtab / lds are done
             if (in_pass && head) qtab[my_slot] = make_uint2(pos_entry, m);
-            __syncthreads();
+            chunk_sync<kWaves>();
             stage_peq<L, W>(lds, qtab, min(nqslots - base_slot, (uint32_t)S::kQSlots), a.peq, lane);
-            __syncthreads();
+            chunk_sync<kWaves>();
             const uint4 *lds_lane = lds + my_slot * (uint32_t)S::kSlotU4 + gl;
             const uint32_t ncols = in_pass ? ncols_live : 0u;
 
@@ -506,12 +529,26 @@ __device__ __forceinline__ bool pull_work(uint32_t *heads, uint32_t n, uint32_t 
 template <int L, int W, bool kCut, bool kPersistent>
 constexpr int kWavesPerEU = L != 1 ? 1 : (W >= 28 ? 3 : ((W >= 20 && !kPersistent && !kCut) ? 4 : 1));
 
+// Issue priority of a variant's waves (s_setprio).  A wave of a long read
+// runs one candidate's chain of dependent column steps, and the chain of the
+// longest candidate sets the time of its batch; so the arbiter of a shared
+// SIMD serves waves of longer reads first and short-read waves fill the other
+// issue slots.  Where every wave of a SIMD is of one kind nothing changes.
+template <int L>
+constexpr int kWavePrio = L >= 64 ? 3 : (L >= 16 ? 2 : (L >= 4 ? 1 : 0));
+
+template <int L>
+__device__ __forceinline__ void set_wave_prio() {
+    if constexpr (kWavePrio<L> > 0) __builtin_amdgcn_s_setprio(kWavePrio<L>);
+}
+
 // Plain launch: a fixed grid strides over the chunks of one slot range.
 template <int L, int W, bool kCut>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<L, W, kCut, false>)))
 void verify_kernel(myers_verify_args_t a) {
     using S = Shape<L, W>;
     __shared__ Workspace<L, W> ws;
+    set_wave_prio<L>();
     uint32_t start = a.start, count = a.count;
     if (a.bin_ranges != nullptr) {
         start = a.bin_ranges[2 * a.variant];
@@ -543,6 +580,7 @@ void verify_persistent_kernel(myers_verify_args_t a) {
     uint32_t *heads = a.queue_heads + a.variant * (uint32_t)(MYERS_QUEUE_PARTS * MYERS_QUEUE_STRIDE);
     uint32_t part = 0;
     if (blockIdx.x >= nchunks) return;      // more workgroups than chunks: the others cover them
+    set_wave_prio<L>();
     zero_row_init<L, W>(ws);
     for (;;) {
         __syncthreads();
@@ -779,6 +817,7 @@ __global__ __launch_bounds__(64 * Q) void verify_wide_kernel(myers_verify_args_t
         start = a.bin_ranges[2 * a.variant];
         count = a.bin_ranges[2 * a.variant + 1];
     }
+    set_wave_prio<64 * Q>();
     zero_row_init_wide<Q, W>(ws);
     uint32_t first = blockIdx.x;
     if ((gridDim.x & 7u) == 0u) first = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
@@ -796,6 +835,7 @@ __global__ __launch_bounds__(64 * Q) void verify_wide_persistent_kernel(myers_ve
     uint32_t *heads = a.queue_heads + a.variant * (uint32_t)(MYERS_QUEUE_PARTS * MYERS_QUEUE_STRIDE);
     uint32_t part = 0;
     if (blockIdx.x >= count) return;        // more workgroups than candidates: the others cover them
+    set_wave_prio<64 * Q>();
     zero_row_init_wide<Q, W>(ws);
     uint32_t staged = 0xFFFFFFFFu;
     for (;;) {
@@ -811,6 +851,78 @@ __global__ __launch_bounds__(64 * Q) void verify_wide_persistent_kernel(myers_ve
         const uint32_t hi = __builtin_amdgcn_readfirstlane(item_hi_sh);
         if (lo == 0xFFFFFFFFu) break;
         verify_wide<Q, W>(a, start, count, lo, hi, 1u, ws, staged);
+    }
+}
+
+// (L, W) pairs compiled into the library.  L = 1 covers queries up to
+// 32*32 = 1024 bases in one lane; L > 1 covers long reads (up to 64*12*32 =
+// 24576 bases).
+#define MYERS_VARIANT_LIST(X)                                                                    \
+    X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 5) X(1, 6) X(1, 7) X(1, 8) X(1, 10) X(1, 12) X(1, 14)   \
+    X(1, 16) X(1, 20) X(1, 24) X(1, 28) X(1, 32)                                                 \
+    X(2, 8) X(2, 12) X(2, 16) X(4, 8) X(4, 12) X(4, 16) X(8, 8) X(8, 12) X(8, 16)                \
+    X(16, 8) X(16, 12) X(16, 16) X(32, 8) X(32, 12) X(32, 16) X(64, 8) X(64, 10) X(64, 12)
+
+// Multi-wave variants (lanes = 64 Q per candidate, one candidate per
+// workgroup of Q waves): long reads with lower per-candidate latency.
+#define MYERS_WIDE_LIST(X) X(2, 4) X(2, 6) X(4, 2) X(4, 3)
+
+// ------------------------------------------------- long-read bins, one launch
+// A bin of long reads holds few candidates, and each takes one chain of
+// dependent column steps: its time is the chain, not the chip's throughput.
+// Separate launches per bin stack such waves on shared SIMDs, where they
+// slow each other.  One launch of MYERS_LONG_WAVES-wave workgroups (the
+// waves of a workgroup sit on different SIMDs of one CU) runs all of them:
+// each wave pulls the next chunk from one counter over the bins in plan
+// order, longest reads first, so the longest chains start first and a SIMD
+// holds one long chain at a time.
+#define MYERS_LONG_CASE(L, W)                                                                    \
+    case ((L) << 8) | (W):                                                                       \
+        if constexpr ((L) >= 4 && (L) <= 64) {                                                   \
+            auto &w = *reinterpret_cast<Workspace<(L), (W)> *>(ws);                              \
+            if (prepared != j) {                                                                 \
+                zero_row_init<(L), (W), MYERS_LONG_WAVES>(w);                                    \
+                prepared = j;                                                                    \
+            }                                                                                    \
+            verify_chunks<(L), (W), false, MYERS_LONG_WAVES>(a, start, count, chunk, chunk + 1u, 1u, w); \
+        }                                                                                        \
+        break;
+
+__global__ __launch_bounds__(64 * MYERS_LONG_WAVES) void verify_long_kernel(myers_verify_args_t a,
+                                                                           myers_long_plan_t p) {
+    extern __shared__ uint4 long_lds[];
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const uint32_t lane = threadIdx.x & 63u;
+    uint4 *ws = long_lds + wave * p.ws_u4;
+    uint32_t *counter = a.queue_heads + MYERS_LONG_HEAD;
+    set_wave_prio<64>();
+    uint32_t prepared = 0xFFFFFFFFu;
+    for (;;) {
+        uint32_t item = 0;
+        if (lane == 0u) item = atomicAdd(counter, 1u);
+        item = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)item, 0));
+        uint32_t j = 0, base = 0, start = 0, count = 0;
+        bool found = false;
+        for (; j < p.n; j++) {
+            const uint32_t v = p.variant[j];
+            const uint32_t c = a.bin_ranges[2u * v + 1u];
+            const uint32_t slots = 64u / (p.key[j] >> 8);
+            const uint32_t chunks = (c + slots - 1u) / slots;
+            if (item - base < chunks) {
+                start = a.bin_ranges[2u * v];
+                count = c;
+                found = true;
+                break;
+            }
+            base += chunks;
+        }
+        if (!found) break;                       // every bin drained
+        const uint32_t chunk = item - base;
+        switch (p.key[j]) {
+            MYERS_VARIANT_LIST(MYERS_LONG_CASE)
+            default:
+                break;
+        }
     }
 }
 
@@ -913,6 +1025,7 @@ __global__ __launch_bounds__(64) void bin_base_kernel(uint32_t nv, uint32_t *ran
     }
     for (uint32_t i = threadIdx.x; i < nv * (uint32_t)MYERS_QUEUE_PARTS; i += blockDim.x)
         queue_heads[i * (uint32_t)MYERS_QUEUE_STRIDE] = 0;
+    if (threadIdx.x == 0) queue_heads[MYERS_LONG_HEAD] = 0;
 }
 
 // Write each candidate's id at its slot: per 256-candidate round, ranks within
@@ -967,18 +1080,6 @@ __global__ __launch_bounds__(256) void bin_scatter_kernel(const uint2 *qinfo, ui
 
 // ------------------------------------------------------------ variant table
 
-// (L, W) pairs compiled into the library.  L = 1 covers queries up to
-// 32*32 = 1024 bases in one lane; L > 1 covers long reads (up to 64*12*32 =
-// 24576 bases).
-#define MYERS_VARIANT_LIST(X)                                                                    \
-    X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 5) X(1, 6) X(1, 7) X(1, 8) X(1, 10) X(1, 12) X(1, 14)   \
-    X(1, 16) X(1, 20) X(1, 24) X(1, 28) X(1, 32)                                                 \
-    X(2, 8) X(2, 12) X(2, 16) X(4, 8) X(4, 12) X(4, 16) X(8, 8) X(8, 12) X(8, 16)                \
-    X(16, 8) X(16, 12) X(16, 16) X(32, 8) X(32, 12) X(32, 16) X(64, 8) X(64, 10) X(64, 12)
-
-// Multi-wave variants (lanes = 64 Q per candidate, one candidate per
-// workgroup of Q waves): long reads with lower per-candidate latency.
-#define MYERS_WIDE_LIST(X) X(2, 4) X(2, 6) X(4, 2) X(4, 3)
 
 namespace {
 struct VariantEntry {
@@ -1009,7 +1110,9 @@ constexpr const void *cut_kernel(bool persistent) {
      reinterpret_cast<const void *>(&verify_wide_persistent_kernel<Q, W>), nullptr, nullptr, 64u * (Q)},
 const VariantEntry kVariants[] = {MYERS_VARIANT_LIST(MYERS_VARIANT_ENTRY) MYERS_WIDE_LIST(MYERS_WIDE_ENTRY)};
 constexpr uint32_t kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
-static_assert(kNumVariants <= MYERS_MAX_VARIANTS, "variant table too large");
+static_assert(kNumVariants < MYERS_MAX_VARIANTS, "variant table too large (the last queue row is the long-read counter)");
+#define MYERS_WS_U4(L, W) (uint32_t)((sizeof(Workspace<L, W>) + 15u) / 16u),
+const uint32_t kWorkspaceU4[] = {MYERS_VARIANT_LIST(MYERS_WS_U4)};
 }  // namespace
 
 extern "C" uint32_t myers_num_variants(void) { return kNumVariants; }
@@ -1056,6 +1159,39 @@ extern "C" int myers_launch_verify(uint32_t v, const myers_verify_args_t *args, 
     const void *fn = cut ? (persistent ? ve.cut_persistent_fn : ve.cut_fn) : (persistent ? ve.persistent_fn : ve.fn);
     hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(ve.block), kargs, 0, stream);
     return (int)e;
+}
+
+extern "C" int myers_variant_long_ok(uint32_t v, uint32_t *ws_u4) {
+    constexpr uint32_t kOneWave = sizeof(kWorkspaceU4) / sizeof(kWorkspaceU4[0]);
+    if (v >= kOneWave || kVariants[v].lanes < 4u || kVariants[v].lanes > 64u) return 0;
+    if (ws_u4) *ws_u4 = kWorkspaceU4[v];
+    return 1;
+}
+
+extern "C" int myers_launch_long(const myers_verify_args_t *args, const myers_long_plan_t *plan, uint32_t grid,
+                                 hipStream_t stream) {
+    if (args == nullptr || plan == nullptr || grid == 0 || plan->n == 0 || plan->n > (uint32_t)MYERS_LONG_MAX ||
+        args->bin_ranges == nullptr || args->queue_heads == nullptr)
+        return -1;
+    uint32_t need = 0;
+    for (uint32_t j = 0; j < plan->n; j++) {
+        uint32_t u4 = 0;
+        if (!myers_variant_long_ok(plan->variant[j], &u4) ||
+            plan->key[j] != ((kVariants[plan->variant[j]].lanes << 8) | kVariants[plan->variant[j]].words))
+            return -1;
+        need = u4 > need ? u4 : need;
+    }
+    if (plan->ws_u4 < need) return -1;
+    const size_t lds = (size_t)plan->ws_u4 * 16u * MYERS_LONG_WAVES;
+    if (lds > 64u * 1024u) {
+        // above 64 KiB a launch needs the kernel's limit raised (gfx950: 160 KiB per workgroup)
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&verify_long_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return (int)e;
+    }
+    void *kargs[] = {const_cast<myers_verify_args_t *>(args), const_cast<myers_long_plan_t *>(plan)};
+    return (int)hipLaunchKernel(reinterpret_cast<const void *>(&verify_long_kernel), dim3(grid),
+                                dim3(64u * MYERS_LONG_WAVES), kargs, lds, stream);
 }
 
 extern "C" int myers_launch_binning(const uint32_t *qinfo, uint32_t num_queries, const void *cands,

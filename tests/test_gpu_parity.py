@@ -31,6 +31,23 @@ def synth_case(seed, ref_size, nq, **kw):
     return ref, b
 
 
+def concat_batches(parts):
+    """One batch from several: Peq entries, reads and candidates appended in
+    order, with posEntry and the candidates' read ids shifted."""
+    peq, qinfo, cands, e_off, q_off = [], [], [], 0, 0
+    for b in parts:
+        qi = b["qinfo"].copy()
+        qi[:, 0] += np.uint32(e_off)
+        c = b["cands"].copy()
+        c["query"] += np.uint32(q_off)
+        peq.append(b["peq"])
+        qinfo.append(qi)
+        cands.append(c)
+        e_off += len(b["peq"])
+        q_off += len(b["qinfo"])
+    return {"peq": np.concatenate(peq), "qinfo": np.concatenate(qinfo), "cands": np.concatenate(cands)}
+
+
 def oracle_for(ref, ref_size, b, distance, mode, max_query_len=0):
     return O.myers_batch(ref, ref_size, b["peq"], b["qinfo"], b["cands"], distance, mode, max_query_len)
 
@@ -295,3 +312,39 @@ def test_multiwave_variants_in_mixed_batches(engine):
         for mode in (M.WINDOW_ALIGNED, M.WINDOW_EXACT):
             got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=mode | engine)
             np.testing.assert_array_equal(got, oracle_for(ref, 3_000_000, b, 0.1, mode))
+
+
+@pytest.mark.parametrize("mode", [M.WINDOW_ALIGNED, M.WINDOW_EXACT])
+def test_long_read_bins_share_one_launch(mode):
+    """Every one-wave long-read variant (4..64 lanes) holds one bin of the same
+    batch, next to short-read bins: the long bins run in the one long-read
+    launch, whose waves switch variants between chunks."""
+    announce(f"long-read launch mode={mode}")
+    ref_size = 3_000_000
+    ref = M.synth_reference(36, ref_size)
+    with M.Myers(ref, ref_size, avg_query=6000, cands_per_query=3, buffer_bytes=256 << 20) as m:
+        variants = m.variants()
+        longs = sorted((v["lanes"] * v["words"], i) for i, v in enumerate(variants) if 4 <= v["lanes"] <= 64)
+        table = m.bin_table()
+        used, lengths = set(), []
+        for cap, i in longs:
+            w = cap
+            while w in used:
+                w -= 1
+            used.add(w)
+            table[w] = i
+            lengths.append(32 * w - 5)
+        lengths += [100, 400, 1000]                       # short-read bins beside them
+        m.set_bin_table(table)
+        b = concat_batches([M.synth_batch(M.make_spec(37 + k, 1, length=n, sub=0.06, ins=0.02, dele=0.02,
+                                                      candidates=2, distance=0.1), ref, ref_size)
+                            for k, n in enumerate(lengths)])
+        assert sorted(b["qinfo"][:, 1].tolist()) == sorted(lengths)
+        got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=mode | M.ENGINE_PERSISTENT)
+        st = m.stats()
+        wq = (b["qinfo"][:, 1].astype(np.int64) + 31) // 32
+        bins = set(int(table[x]) for x in wq)
+        nlong = sum(1 for v in bins if 4 <= variants[v]["lanes"] <= 64)
+        assert nlong >= 10
+        assert st.launches == len(bins) - nlong + 1
+        np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b, 0.1, mode))

@@ -61,6 +61,7 @@ typedef struct myers_ctx {
     int       occupancy[MYERS_MAX_VARIANTS];       /* plain kernel, workgroups per CU */
     int       occupancy_p[MYERS_MAX_VARIANTS];     /* persistent kernel */
     uint32_t  chunk_factor[MYERS_MAX_VARIANTS];    /* persistent: chunks per queue pull */
+    uint32_t  long_seq;                            /* long-read launches so far (placement rotation) */
     uint8_t   bin_of_words[MAX_QUERY_WORDS + 1];
     uint8_t  *d_bin_of_words;
     uint64_t  tune_gen;          /* bumped when the bin table or chunk factors change */
@@ -621,8 +622,10 @@ static uint32_t present_by_cost(const myers_ctx_t *ctx, uint64_t present, uint32
  * first, when their chunks (estimated from the reads per bin and the batch's
  * candidates per read) fit a few rounds of one wave per SIMD.  Such bins are
  * bound by each candidate's chain of column steps; larger long-read batches
- * are throughput work and keep one launch per bin.  Returns the bins taken
- * (0: none). */
+ * are throughput work and keep one launch per bin.  While one round of waves
+ * still holds everything, the longest bins (over 256 words) run each
+ * candidate on a whole 4-wave workgroup, which shortens its chain.  Returns
+ * the bins taken (0: none). */
 #define LONG_ROUNDS 4u
 static uint32_t long_plan(const myers_ctx_t *ctx, const bin_scan_t *scan, const batch_t *bt,
                           const uint32_t *order, uint32_t n, myers_long_plan_t *lp)
@@ -630,22 +633,44 @@ static uint32_t long_plan(const myers_ctx_t *ctx, const bin_scan_t *scan, const 
     memset(lp, 0, sizeof(*lp));
     if (!scan->sized || bt->num_queries == 0) return 0;
     const double cpr = (double)bt->num_cands / (double)bt->num_queries;
-    double items = 0.0;
-    for (uint32_t i = 0; i < n; i++) {
+    const double simds = (double)MYERS_LONG_WAVES * (double)ctx->num_cus;
+    uint32_t take[MYERS_LONG_MAX], nt = 0;
+    double waves = 0.0;
+    for (uint32_t i = 0; i < n && nt < MYERS_LONG_MAX; i++) {
         const uint32_t v = order[i];
+        if (!myers_variant_long_ok(v, NULL)) continue;
+        take[nt++] = v;
+        waves += (double)scan->reads[v] * cpr / (double)ctx->variants[v].slots;
+    }
+    if (nt == 0 || waves > (double)LONG_ROUNDS * simds) return 0;
+    /* order[] is longest first: widen bins while one round still fits */
+    uint32_t nw = 0;
+    for (; nw < nt; nw++) {
+        const uint32_t v = take[nw];
+        const uint32_t words = ctx->variants[v].total_words;
+        if (words <= 64u * MYERS_LONG_WAVES || words > 64u * MYERS_LONG_WAVES * MYERS_WIDE_MAX_WORDS) break;
+        const double cands = (double)scan->reads[v] * cpr;
+        const double more = cands * (double)MYERS_LONG_WAVES - cands / (double)ctx->variants[v].slots;
+        if (waves + more > simds) break;
+        waves += more;
+        lp->wide_variant[nw] = v;
+        lp->wide_words[nw] = (words + 64u * MYERS_LONG_WAVES - 1u) / (64u * MYERS_LONG_WAVES);
+    }
+    lp->n_wide = nw;
+    for (uint32_t k = nw; k < nt; k++) {
+        const uint32_t v = take[k];
         uint32_t u4 = 0;
-        if (!myers_variant_long_ok(v, &u4) || lp->n == MYERS_LONG_MAX) continue;
+        myers_variant_long_ok(v, &u4);
         lp->variant[lp->n] = v;
         lp->key[lp->n] = (ctx->variants[v].lanes << 8) | ctx->variants[v].words;
         lp->n++;
         if (u4 > lp->ws_u4) lp->ws_u4 = u4;
-        items += (double)scan->reads[v] * cpr / (double)ctx->variants[v].slots;
     }
-    if (lp->n == 0 || items > (double)(LONG_ROUNDS * MYERS_LONG_WAVES) * (double)ctx->num_cus) {
-        lp->n = 0;
-        return 0;
+    if (nw) {
+        const uint32_t u4 = (myers_long_wide_lds_bytes() + 16u * MYERS_LONG_WAVES - 1u) / (16u * MYERS_LONG_WAVES);
+        if (u4 > lp->ws_u4) lp->ws_u4 = u4;
     }
-    return lp->n;
+    return nt;
 }
 
 /* Enqueue the whole on-device pipeline of one batch on b->stream (no timing
@@ -717,7 +742,8 @@ static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan
         uint32_t rest[MYERS_MAX_VARIANTS], nrest = 0;
         for (uint32_t i = 0; i < n; i++) {
             int in_long = 0;
-            for (uint32_t j = 0; j < nlong; j++) in_long |= lp.variant[j] == order[i];
+            for (uint32_t j = 0; j < lp.n; j++) in_long |= lp.variant[j] == order[i];
+            for (uint32_t j = 0; j < lp.n_wide; j++) in_long |= lp.wide_variant[j] == order[i];
             if (!in_long) rest[nrest++] = order[i];
         }
         launches = nrest + (nlong ? 1u : 0u);
@@ -726,6 +752,11 @@ static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan
         HIP_TRY(hipEventRecord(b->ev_fork, b->stream));
         for (uint32_t j = 0; j < width; j++) HIP_TRY(hipStreamWaitEvent(b->side[j], b->ev_fork, 0));
         if (nlong) {
+            /* successive launches start their first items a golden-ratio step
+             * of workgroups apart (overlapping batches use other CUs) */
+            const uint32_t seq = __atomic_fetch_add(&ctx->long_seq, 1u, __ATOMIC_RELAXED);
+            const uint32_t wg = (uint32_t)(((uint64_t)seq * 2654435769ull & 0xFFFFFFFFull) * ctx->num_cus >> 32);
+            lp.rotate = wg * MYERS_LONG_WAVES;
             rc = myers_launch_long(&a, &lp, ctx->num_cus, b->side[0]);
             if (rc) return hip_fail((hipError_t)rc, "long-read launch", __FILE__, __LINE__);
         }
@@ -779,7 +810,9 @@ static int enqueue_verify(myers_buffer_t *b, const batch_t *bt, const qryInfo_t 
     memcpy(&thr_bits, &bt->threshold, sizeof(thr_bits));
     uint32_t order[MYERS_MAX_VARIANTS];
     myers_long_plan_t lp;
-    const uint64_t plan_key = long_plan(b->ctx, &scan, bt, order, present_by_cost(b->ctx, scan.present, order), &lp);
+    const uint64_t plan_key = (uint64_t)long_plan(b->ctx, &scan, bt, order,
+                                                  present_by_cost(b->ctx, scan.present, order), &lp) |
+                              ((uint64_t)lp.n_wide << 32);
     const uint64_t key[12] = {(uint64_t)(uintptr_t)bt->d_peq, (uint64_t)(uintptr_t)bt->d_qinfo,
                               (uint64_t)(uintptr_t)bt->d_cands, (uint64_t)(uintptr_t)bt->d_results,
                               ((uint64_t)bt->num_entries << 32) | bt->num_queries,

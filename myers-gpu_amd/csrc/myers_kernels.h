@@ -71,16 +71,32 @@ int myers_variant_occupancy_persistent(uint32_t v, int *wg_per_cu);
 
 /* Long-read bins in one launch.  Bins of one-wave variants with 4..64 lanes
  * per candidate (myers_variant_long_ok) can run together in workgroups of
- * MYERS_LONG_WAVES independent waves; each wave pulls the next chunk from
- * one counter (queue_heads + MYERS_LONG_HEAD, reset by the binning) that
- * runs over the plan's bins in order.  Needs bin_ranges and queue_heads. */
+ * MYERS_LONG_WAVES waves: first the plan's wide bins, each candidate on a
+ * whole workgroup, then the others, each wave on its own.  Items (chunks of the plan's bins, in
+ * plan order) 0 .. waves-1 go one to a wave, starting at wave `rotate`, so
+ * that successive launches start their longest items on other workgroups;
+ * later items are pulled from one counter (queue_heads + MYERS_LONG_HEAD,
+ * reset by the binning).  Needs bin_ranges and queue_heads. */
 enum { MYERS_LONG_MAX = 16, MYERS_LONG_WAVES = 4 };
 #define MYERS_LONG_HEAD ((MYERS_MAX_VARIANTS - 1) * MYERS_QUEUE_PARTS * MYERS_QUEUE_STRIDE)
+#define MYERS_WIDE_HEAD (MYERS_LONG_HEAD + MYERS_QUEUE_STRIDE)
+/* LDS of one workgroup-wide candidate of the long-read launch (bytes), and
+ * the most words per lane it takes: reads up to 64 * MYERS_LONG_WAVES * 32 *
+ * MYERS_WIDE_MAX_WORDS bases. */
+enum { MYERS_WIDE_MAX_WORDS = 3 };
+uint32_t myers_long_wide_lds_bytes(void);
 typedef struct {
-    uint32_t n;                          /* bins, in pull order */
+    uint32_t n;                          /* one-wave bins, in pull order */
     uint32_t ws_u4;                      /* LDS per wave in 16-byte units (max over the bins) */
+    uint32_t rotate;                     /* first-round item of wave s is (s - rotate) mod waves */
     uint32_t variant[MYERS_LONG_MAX];    /* variant ids (rows of bin_ranges) */
     uint32_t key[MYERS_LONG_MAX];        /* lanes << 8 | words of each */
+    /* bins run first, one candidate per workgroup (MYERS_LONG_WAVES waves of
+     * 64 lanes x wide_words[j] words); the first round starts at workgroup
+     * rotate / MYERS_LONG_WAVES, later ones pull queue_heads + MYERS_WIDE_HEAD */
+    uint32_t n_wide;
+    uint32_t wide_variant[MYERS_LONG_MAX];
+    uint32_t wide_words[MYERS_LONG_MAX]; /* 2 or 3 */
 } myers_long_plan_t;
 /* 1 if variant v can run in the long-read launch, and its LDS per wave. */
 int myers_variant_long_ok(uint32_t v, uint32_t *ws_u4);

@@ -655,6 +655,106 @@ __device__ __forceinline__ void stage_peq_wide(uint4 *lds, uint2 qi, const uint3
     }
 }
 
+// One candidate's columns on wave wq of its workgroup (Q > 1).  A wave's
+// role is fixed for the whole loop, so each role is its own instantiation:
+// kBelow -- lane 0's hin is the top lane of the wave below, read from the LDS
+// ring; kAbove -- this wave's top lane feeds the wave above; kTop -- the top
+// wave keeps the score row.  Ring word of block b: the top lane's Ph bit 31
+// of step 8b+i at bit i, its Mh bit 31 at bit 8+i.  Returns {min_col,
+// min_score}, meaningful on lane 63 of the top wave.
+template <int Q, int W, bool kBelow, bool kAbove, bool kTop>
+__device__ __forceinline__ uint2 wide_columns(WideWorkspace<Q, W> &ws, uint32_t wq, uint32_t tid,
+                                              const uint32_t *__restrict__ ref, int32_t last_word,
+                                              uint64_t nib0, uint32_t lag, uint32_t ncols, uint32_t m) {
+    using S = WideShape<Q, W>;
+    constexpr uint32_t kGS = (uint32_t)(S::kRows * S::kRowU4);
+    const uint32_t lane = tid & 63u;
+    const uint4 *lds_lane = ws.lds + tid;
+    const uint32_t nblocks = (ncols + S::kMaxLag + 7u) >> 3;
+    const int64_t qn = (int64_t)nib0 - (int64_t)lag;
+    const int32_t w0 = (int32_t)(qn >> 3);
+    const uint32_t shift = (uint32_t)(qn & 7) * 4u;
+
+    uint32_t Pv[W], Mv[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        Pv[k] = ~0u;
+        Mv[k] = 0u;
+    }
+    int32_t score = (int32_t)m, min_score = (int32_t)m;
+    uint32_t min_col = 0, ph_top = 0, mh_top = 0, acc_p = 0, acc_m = 0;
+    auto text_word = [&](int32_t k) { return ref[min(max(w0 + k, 0), last_word)]; };
+    auto text_block = [&](uint32_t lo, uint32_t hi, uint32_t b) {
+        uint32_t t = sanitize_text(__builtin_amdgcn_alignbit(hi, lo, shift));
+        // steps before this lane's first column read the zero row
+        const int32_t early = (int32_t)lag - 8 * (int32_t)b;
+        if (early > 0) {
+            const uint32_t low = early >= 8 ? ~0u : ((1u << (4 * early)) - 1u);
+            t = (t & ~low) | (0x55555555u & low);
+        }
+        return t;
+    };
+    const uint32_t tw0 = text_word(0);
+    const uint32_t tw1 = text_word(1);
+    uint32_t tw2 = text_word(2);
+    uint32_t tw3 = text_word(3);
+    uint32_t text = text_block(tw0, tw1, 0);
+    uint32_t text_next = text_block(tw1, tw2, 1);
+    uint32_t cur_off = (text & 7u) * (uint32_t)S::kRowU4;
+    uint4 eq4 = lds_lane[cur_off];
+    for (uint32_t b = 0; b < nblocks; b++) {
+        const uint32_t tw4 = text_word((int32_t)b + 4);
+        // ring words of the wave below: its top lane's steps 8b-9 .. 8b-2,
+        // written in blocks b-2 and b-1 (the same value in every lane)
+        uint32_t w1 = 0, w2 = 0;
+        if constexpr (kBelow) {
+            if (b >= 1u) w1 = ws.ring[wq - 1u][(b - 1u) % kRingWords];
+            if (b >= 2u) w2 = ws.ring[wq - 1u][(b - 2u) % kRingWords];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint32_t cn = (i < 7) ? ((text >> (4 * (i + 1))) & 7u) : (text_next & 7u);
+            const uint32_t next_off = cn * (uint32_t)S::kRowU4;
+            const uint4 eq4n = lds_lane[next_off];
+            // hin from the lane below (DPP wave_shr:1); lane 0 keeps the fill:
+            // 0 on the bottom wave, else the wave below's bit of step 8b+i-9
+            // moved to bit 31 (only bit 31 of a hin is read)
+            uint32_t fill_p = 0u, fill_m = 0u;
+            if constexpr (kBelow) {
+                fill_p = i == 0 ? (w2 << 24) : (w1 << (32 - i));
+                fill_m = i == 0 ? (w2 << 16) : (w1 << (24 - i));
+            }
+            const uint32_t ph_in = (uint32_t)__builtin_amdgcn_update_dpp((int)fill_p, (int)ph_top, 0x138, 0xF, 0xF, false);
+            const uint32_t mh_in = (uint32_t)__builtin_amdgcn_update_dpp((int)fill_m, (int)mh_top, 0x138, 0xF, 0xF, false);
+            column_grp<W, true, false, kGS>(Pv, Mv, eq4, lds_lane, cur_off, 0u, ph_in, mh_in, ph_top, mh_top);
+            eq4 = eq4n;
+            if constexpr (kAbove) {
+                acc_p = (acc_p >> 1) | (ph_top & 0x80000000u);
+                acc_m = (acc_m >> 1) | (mh_top & 0x80000000u);
+            }
+            if constexpr (kTop) {
+                // score row: the top lane of the top wave, at column t - kMaxLag
+                score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
+                const uint32_t col = 8u * b + (uint32_t)i - S::kMaxLag;
+                const bool better = (score < min_score) && (col < ncols);
+                min_col = better ? col : min_col;
+                min_score = better ? score : min_score;
+            }
+            cur_off = next_off;
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if constexpr (kAbove) {
+            if (lane == 63u) ws.ring[wq][b % kRingWords] = (acc_p >> 24) | ((acc_m >> 16) & 0xFF00u);
+        }
+        __syncthreads();
+        text = text_next;
+        text_next = text_block(tw2, tw3, b + 2);
+        tw2 = tw3;
+        tw3 = tw4;
+    }
+    return make_uint2(min_col, (uint32_t)min_score);
+}
+
 // Candidates cand_lo, cand_lo + stride, ... < cand_hi of the slot range
 // [start, start + count), one per workgroup at a time.
 template <int Q, int W>
@@ -662,11 +762,9 @@ __device__ __forceinline__ void verify_wide(const myers_verify_args_t &a, uint32
                                             uint32_t cand_lo, uint32_t cand_hi, uint32_t stride,
                                             WideWorkspace<Q, W> &ws, uint32_t &staged_query) {
     using S = WideShape<Q, W>;
-    constexpr uint32_t kGS = (uint32_t)(S::kRows * S::kRowU4);
     const uint32_t tid = threadIdx.x;
     const uint32_t wq = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave of the workgroup
     const uint32_t lag = tid + kWaveLag * wq;                      // this lane's column lag
-    const bool bottom = (tid & 63u) == 0u;
     const uint32_t *__restrict__ ref = a.ref;
     const int32_t last_word = (int32_t)(a.ref_words - 1);
     const CandRec *__restrict__ cands = static_cast<const CandRec *>(a.cands);
@@ -708,91 +806,16 @@ __device__ __forceinline__ void verify_wide(const myers_verify_args_t &a, uint32
             __syncthreads();
             staged_query = cr.query;
         }
-        const uint4 *lds_lane = ws.lds + tid;
-        const uint32_t nblocks = (ncols + S::kMaxLag + 7u) >> 3;
-        const int64_t qn = (int64_t)nib0 - (int64_t)lag;
-        const int32_t w0 = (int32_t)(qn >> 3);
-        const uint32_t shift = (uint32_t)(qn & 7) * 4u;
-
-        uint32_t Pv[W], Mv[W];
-#pragma unroll
-        for (int k = 0; k < W; k++) {
-            Pv[k] = ~0u;
-            Mv[k] = 0u;
-        }
-        int32_t score = (int32_t)m, min_score = (int32_t)m;
-        uint32_t min_col = 0, ph_top = 0, mh_top = 0;
-        auto text_word = [&](int32_t k) { return ref[min(max(w0 + k, 0), last_word)]; };
-        auto text_block = [&](uint32_t lo, uint32_t hi, uint32_t b) {
-            uint32_t t = sanitize_text(__builtin_amdgcn_alignbit(hi, lo, shift));
-            // steps before this lane's first column read the zero row
-            const int32_t early = (int32_t)lag - 8 * (int32_t)b;
-            if (early > 0) {
-                const uint32_t low = early >= 8 ? ~0u : ((1u << (4 * early)) - 1u);
-                t = (t & ~low) | (0x55555555u & low);
-            }
-            return t;
-        };
-        const uint32_t tw0 = text_word(0);
-        const uint32_t tw1 = text_word(1);
-        uint32_t tw2 = text_word(2);
-        uint32_t tw3 = text_word(3);
-        uint32_t text = text_block(tw0, tw1, 0);
-        uint32_t text_next = text_block(tw1, tw2, 1);
-        uint32_t cur_off = (text & 7u) * (uint32_t)S::kRowU4;
-        uint4 eq4 = lds_lane[cur_off];
-        for (uint32_t b = 0; b < nblocks; b++) {
-            const uint32_t tw4 = text_word((int32_t)b + 4);
-            // hin bits of this block's 8 steps from the wave below: its top
-            // lane's steps 8b-9 .. 8b-2, written in blocks b-2 and b-1
-            uint32_t w1 = 0, w2 = 0;
-            if (Q > 1 && wq > 0u) {
-                if (b >= 1u) w1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)ws.ring[wq - 1u][(b - 1u) % kRingWords]);
-                if (b >= 2u) w2 = (uint32_t)__builtin_amdgcn_readfirstlane((int)ws.ring[wq - 1u][(b - 2u) % kRingWords]);
-            }
-            uint32_t acc = 0;
-#pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const uint32_t cn = (i < 7) ? ((text >> (4 * (i + 1))) & 7u) : (text_next & 7u);
-                const uint32_t next_off = cn * (uint32_t)S::kRowU4;
-                const uint4 eq4n = lds_lane[next_off];
-                uint32_t ph_in = from_lower_lane<64>(ph_top, ~0u);
-                uint32_t mh_in = from_lower_lane<64>(mh_top, ~0u);
-                if (Q > 1 && wq > 0u) {
-                    const uint32_t bits = (i == 0) ? (w2 >> 14) : (w1 >> (2 * (i - 1)));
-                    ph_in = bottom ? (bits << 31) : ph_in;
-                    mh_in = bottom ? ((bits >> 1) << 31) : mh_in;
-                }
-                column_grp<W, true, false, kGS>(Pv, Mv, eq4, lds_lane, cur_off, 0u, ph_in, mh_in, ph_top, mh_top);
-                eq4 = eq4n;
-                if (Q > 1 && wq + 1u < (uint32_t)Q) {
-                    // (the builtin returns int: shift as unsigned)
-                    const uint32_t pt = (uint32_t)__builtin_amdgcn_readlane((int)ph_top, 63) >> 31;
-                    const uint32_t mt = (uint32_t)__builtin_amdgcn_readlane((int)mh_top, 63) >> 31;
-                    acc |= (pt | (mt << 1)) << (2 * i);
-                }
-                // score row: the top lane of the top wave, at column t - kMaxLag
-                score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
-                const uint32_t col = 8u * b + (uint32_t)i - S::kMaxLag;
-                const bool better = (score < min_score) && (col < ncols);
-                min_col = better ? col : min_col;
-                min_score = better ? score : min_score;
-                cur_off = next_off;
-                __builtin_amdgcn_sched_barrier(0);
-            }
-            if (Q > 1) {
-                if (wq + 1u < (uint32_t)Q && bottom) ws.ring[wq][b % kRingWords] = acc;
-                __syncthreads();
-            }
-            text = text_next;
-            text_next = text_block(tw2, tw3, b + 2);
-            tw2 = tw3;
-            tw3 = tw4;
+        uint2 r;
+        if (wq == 0u) {
+            r = wide_columns<Q, W, false, true, false>(ws, wq, tid, ref, last_word, nib0, lag, ncols, m);
+        } else if (wq + 1u < (uint32_t)Q) {
+            r = wide_columns<Q, W, true, true, false>(ws, wq, tid, ref, last_word, nib0, lag, ncols, m);
+        } else {
+            r = wide_columns<Q, W, true, false, true>(ws, wq, tid, ref, last_word, nib0, lag, ncols, m);
         }
         if (tid == (uint32_t)S::kLanes - 1u) {
-            uint2 r = make_uint2(min_col, (uint32_t)min_score);
-            if (a.threshold >= 0.0f && (uint32_t)min_score > threshold_of(m, a.threshold))
-                r = make_uint2(kNoMatch, kNoMatch);
+            if (a.threshold >= 0.0f && r.y > threshold_of(m, a.threshold)) r = make_uint2(kNoMatch, kNoMatch);
             results[cand] = r;
         }
     }
@@ -888,19 +911,71 @@ __global__ __launch_bounds__(64 * Q) void verify_wide_persistent_kernel(myers_ve
         }                                                                                        \
         break;
 
+using LongWide = WideWorkspace<MYERS_LONG_WAVES, MYERS_WIDE_MAX_WORDS>;
+static_assert(sizeof(WideWorkspace<MYERS_LONG_WAVES, 2>) == sizeof(LongWide) &&
+                  WideShape<MYERS_LONG_WAVES, 2>::kGroups == WideShape<MYERS_LONG_WAVES, 3>::kGroups,
+              "wide candidates of 2 and 3 words share one LDS layout");
+
 __global__ __launch_bounds__(64 * MYERS_LONG_WAVES) void verify_long_kernel(myers_verify_args_t a,
                                                                            myers_long_plan_t p) {
     extern __shared__ uint4 long_lds[];
+    __shared__ uint32_t wide_item_sh;
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const uint32_t lane = threadIdx.x & 63u;
+    set_wave_prio<64>();
+    // Phase 1: the wide bins, one candidate per workgroup (longest reads).
+    if (p.n_wide) {
+        LongWide &wws = *reinterpret_cast<LongWide *>(long_lds);
+        zero_row_init_wide<MYERS_LONG_WAVES, MYERS_WIDE_MAX_WORDS>(wws);
+        uint32_t staged = 0xFFFFFFFFu;
+        const uint32_t grid = gridDim.x;
+        uint32_t witem = (blockIdx.x + grid - (p.rotate / (uint32_t)MYERS_LONG_WAVES) % grid) % grid;
+        for (bool first = true;; first = false) {
+            if (!first) {
+                if (threadIdx.x == 0) wide_item_sh = atomicAdd(a.queue_heads + MYERS_WIDE_HEAD, 1u);
+                __syncthreads();
+                witem = grid + (uint32_t)__builtin_amdgcn_readfirstlane((int)wide_item_sh);
+                __syncthreads();                 // every wave has read it before the next pull
+            }
+            uint32_t j = 0, base = 0, start = 0, count = 0;
+            bool found = false;
+            for (; j < p.n_wide; j++) {
+                const uint32_t v = p.wide_variant[j];
+                const uint32_t c = a.bin_ranges[2u * v + 1u];
+                if (witem - base < c) {
+                    start = a.bin_ranges[2u * v];
+                    count = c;
+                    found = true;
+                    break;
+                }
+                base += c;
+            }
+            if (!found) break;                   // every wide bin drained
+            const uint32_t slot = witem - base;
+            if (p.wide_words[j] == 2u) {
+                verify_wide<MYERS_LONG_WAVES, 2>(a, start, count, slot, slot + 1u, 1u,
+                                                 *reinterpret_cast<WideWorkspace<MYERS_LONG_WAVES, 2> *>(long_lds),
+                                                 staged);
+            } else {
+                verify_wide<MYERS_LONG_WAVES, 3>(a, start, count, slot, slot + 1u, 1u, wws, staged);
+            }
+        }
+        __syncthreads();                         // phase 2 reuses the LDS
+    }
+    // Phase 2: the other bins, one chunk per wave.
     uint4 *ws = long_lds + wave * p.ws_u4;
     uint32_t *counter = a.queue_heads + MYERS_LONG_HEAD;
-    set_wave_prio<64>();
     uint32_t prepared = 0xFFFFFFFFu;
-    for (;;) {
-        uint32_t item = 0;
-        if (lane == 0u) item = atomicAdd(counter, 1u);
-        item = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)item, 0));
+    // first round: one item per wave, placed from wave `rotate` on; then pulls
+    const uint32_t nw = gridDim.x * (uint32_t)MYERS_LONG_WAVES;
+    const uint32_t slot = blockIdx.x * (uint32_t)MYERS_LONG_WAVES + wave;
+    uint32_t item = (slot + nw - p.rotate % nw) % nw;
+    for (bool first = true;; first = false) {
+        if (!first) {
+            uint32_t got = 0;
+            if (lane == 0u) got = atomicAdd(counter, 1u);
+            item = nw + (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)got, 0));
+        }
         uint32_t j = 0, base = 0, start = 0, count = 0;
         bool found = false;
         for (; j < p.n; j++) {
@@ -1025,7 +1100,10 @@ __global__ __launch_bounds__(64) void bin_base_kernel(uint32_t nv, uint32_t *ran
     }
     for (uint32_t i = threadIdx.x; i < nv * (uint32_t)MYERS_QUEUE_PARTS; i += blockDim.x)
         queue_heads[i * (uint32_t)MYERS_QUEUE_STRIDE] = 0;
-    if (threadIdx.x == 0) queue_heads[MYERS_LONG_HEAD] = 0;
+    if (threadIdx.x == 0) {
+        queue_heads[MYERS_LONG_HEAD] = 0;
+        queue_heads[MYERS_WIDE_HEAD] = 0;
+    }
 }
 
 // Write each candidate's id at its slot: per 256-candidate round, ranks within
@@ -1168,12 +1246,22 @@ extern "C" int myers_variant_long_ok(uint32_t v, uint32_t *ws_u4) {
     return 1;
 }
 
+extern "C" uint32_t myers_long_wide_lds_bytes(void) { return (uint32_t)sizeof(LongWide); }
+
 extern "C" int myers_launch_long(const myers_verify_args_t *args, const myers_long_plan_t *plan, uint32_t grid,
                                  hipStream_t stream) {
-    if (args == nullptr || plan == nullptr || grid == 0 || plan->n == 0 || plan->n > (uint32_t)MYERS_LONG_MAX ||
+    if (args == nullptr || plan == nullptr || grid == 0 || plan->n > (uint32_t)MYERS_LONG_MAX ||
+        plan->n_wide > (uint32_t)MYERS_LONG_MAX || plan->n + plan->n_wide == 0 ||
         args->bin_ranges == nullptr || args->queue_heads == nullptr)
         return -1;
     uint32_t need = 0;
+    for (uint32_t j = 0; j < plan->n_wide; j++) {
+        const uint32_t v = plan->wide_variant[j], w = plan->wide_words[j];
+        if (!myers_variant_long_ok(v, nullptr) || (w != 2u && w != 3u) ||
+            kVariants[v].lanes * kVariants[v].words > 64u * MYERS_LONG_WAVES * w)
+            return -1;
+        need = (uint32_t)((sizeof(LongWide) + 16u * MYERS_LONG_WAVES - 1u) / (16u * MYERS_LONG_WAVES));
+    }
     for (uint32_t j = 0; j < plan->n; j++) {
         uint32_t u4 = 0;
         if (!myers_variant_long_ok(plan->variant[j], &u4) ||

@@ -348,3 +348,26 @@ def test_long_read_bins_share_one_launch(mode):
         assert nlong >= 10
         assert st.launches == len(bins) - nlong + 1
         np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b, 0.1, mode))
+
+
+def test_long_read_launch_wide_rounds():
+    """Long reads with far more candidates than the batch average: the host's
+    estimate puts their bin on whole workgroups, and the device finds more
+    candidates than workgroups, so later rounds pull from the wide counter."""
+    announce("long-read launch, wide rounds")
+    ref_size = 3_000_000
+    ref = M.synth_reference(38, ref_size)
+    parts = [M.synth_batch(M.make_spec(39, 1500, length=100, sub=0.02, ins=0.005, dele=0.005, candidates=1,
+                                       distance=0.1), ref, ref_size),
+             M.synth_batch(M.make_spec(40, 20, length=9000, sub=0.06, ins=0.02, dele=0.02, candidates=20,
+                                       distance=0.1), ref, ref_size),
+             M.synth_batch(M.make_spec(41, 20, length=4000, sub=0.06, ins=0.02, dele=0.02, candidates=4,
+                                       distance=0.1), ref, ref_size)]
+    b = concat_batches(parts)
+    with M.Myers(ref, ref_size, avg_query=400, cands_per_query=2, buffer_bytes=256 << 20) as m:
+        got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=M.ENGINE_PERSISTENT)
+        st = m.stats()
+        table = m.bin_table()
+        bins = set(int(table[x]) for x in (b["qinfo"][:, 1].astype(np.int64) + 31) // 32)
+        assert st.launches == len(bins) - 2 + 1          # the two long bins share one launch
+        np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b, 0.1, O.MODE_ALIGNED))

@@ -265,7 +265,8 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay the batch pipeline as a captured HIP graph")
     ap.add_argument("--compare-engines", action="store_true", help="also time the other engine on the same batch")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
-    ap.add_argument("--serve-loads", default="0.7", help="serving load levels (fractions of saturation); '' = skip")
+    ap.add_argument("--serve-loads", default="0.1,0.3,0.5,0.7,1.0",
+                    help="serving load levels (fractions of saturation); '' = skip")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--serve-buffers", type=int, default=3)
     ap.add_argument("--serve-batch-cells", type=float, default=0, help="serving batch cap on cells (0 = library default)")

@@ -134,8 +134,8 @@ uint64_t    myersGetReferenceSize(void *myersBuffer);             /* extension *
  * copies; the batch runs on the buffer's stream and uses its binning
  * workspace (so it needs numCandidates <= getMaxCandidates(buffer)).
  * h_qinfo (optional, host copy of the qinfo array) lets the library pick
- * the kernel variants without reading device memory; NULL launches every
- * variant over the device-side bins.  Candidates naming a query >=
+ * the kernel variants, and plan the long-read launch, without reading device
+ * memory; NULL launches every variant over the device-side bins.  Candidates naming a query >=
  * numQueries, or a query whose entries run past numPEQEntries, get the skip
  * sentinel.  Returns after enqueueing. */
 int myersVerifyDevice(void *myersBuffer, const void *d_peq, uint32_t numPEQEntries, const void *d_qinfo,

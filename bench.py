@@ -263,7 +263,9 @@ def main():
     ap.add_argument("--engine", default="auto", choices=["auto", "plain", "persistent"])
     ap.add_argument("--mode", default="aligned", choices=["aligned", "exact"])
     ap.add_argument("--graph", action="store_true", help="replay the batch pipeline as a captured HIP graph")
-    ap.add_argument("--compare-engines", action="store_true", help="also time the other engine on the same batch")
+    ap.add_argument("--compare-engines", action="store_true",
+                    help="also time the plain and persistent engines, and one plain launch of the variant "
+                         "for the longest read, on the same batch")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--serve-loads", default="0.1,0.3,0.5,0.7,1.0",
                     help="serving load levels (fractions of saturation); '' = skip")
@@ -320,11 +322,26 @@ def main():
             if not np.array_equal(bufs[3].get(), results):
                 raise SystemExit(f"bench: engine {name} differs from the timed run")
             other[name] = r
+        # the plain kernel sized for the longest read, one launch over the whole batch
+        table = m.bin_table()
+        wmax = int((int(b["qinfo"][:, 1].max()) + 31) // 32)
+        v_long = int(table[wmax])
+        forced = table.copy()
+        forced[1:wmax + 1] = v_long
+        m.set_bin_table(forced)
+        one = run_steps(m, bufs, b, mode_flag | engines["plain"], max(3, args.steps // 2), 1)
+        if not np.array_equal(bufs[3].get(), results):
+            raise SystemExit("bench: the one-variant launch differs from the timed run")
+        m.set_bin_table(table)
+        vinfo = m.variants()[v_long]
         comparison = {"plain_candidates_per_s": other["plain"]["cand_per_s"],
                       "persistent_candidates_per_s": other["persistent"]["cand_per_s"],
                       "plain_kernel_ms": other["plain"]["kernel_ms"],
                       "persistent_kernel_ms": other["persistent"]["kernel_ms"],
-                      "persistent_over_plain": other["persistent"]["cand_per_s"] / other["plain"]["cand_per_s"]}
+                      "persistent_over_plain": other["persistent"]["cand_per_s"] / other["plain"]["cand_per_s"],
+                      "one_launch_variant": f"L{vinfo['lanes']}xW{vinfo['words']}",
+                      "one_launch_candidates_per_s": one["cand_per_s"],
+                      "persistent_over_one_launch": other["persistent"]["cand_per_s"] / one["cand_per_s"]}
         log(f"engines: {comparison}")
 
     elapsed, kernel_ms = run["elapsed"], run["kernel_ms"]

@@ -371,3 +371,35 @@ def test_long_read_launch_wide_rounds():
         bins = set(int(table[x]) for x in (b["qinfo"][:, 1].astype(np.int64) + 31) // 32)
         assert st.launches == len(bins) - 2 + 1          # the two long bins share one launch
         np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b, 0.1, O.MODE_ALIGNED))
+
+
+def test_graph_replays_long_read_launch():
+    """The long-read launch (wide phase and one-wave phase) inside a captured
+    graph: every replay, with eager work in between, matches the eager run."""
+    announce("graph replays, long-read launch")
+    ref_size = 3_000_000
+    ref = M.synth_reference(42, ref_size)
+    b = concat_batches([M.synth_batch(M.make_spec(43 + k, n, length=length, sub=0.05, ins=0.02, dele=0.02,
+                                                  candidates=c, distance=0.1), ref, ref_size)
+                        for k, (n, length, c) in enumerate([(200, 300, 6), (6, 3000, 3), (4, 9000, 3),
+                                                            (3, 17000, 2)])])
+    with M.Myers(ref, ref_size, avg_query=600, cands_per_query=5, buffer_bytes=256 << 20) as m:
+        dp, dq, dc = M.DeviceArray(b["peq"]), M.DeviceArray(b["qinfo"]), M.DeviceArray(b["cands"])
+        nq, nc, ne = len(b["qinfo"]), len(b["cands"]), len(b["peq"])
+        dr = M.DeviceArray.empty((nc, 2), np.uint32)
+        M.verify_device(m, dp, ne, dq, nq, dc, nc, 0.1, dr, flags=M.ENGINE_PERSISTENT, h_qinfo=b["qinfo"])
+        M.synchronize(m)
+        eager = dr.get()
+        assert m.stats().launches < len(set(m.bin_table()[(b["qinfo"][:, 1].astype(np.int64) + 31) // 32]))
+        np.testing.assert_array_equal(eager, oracle_for(ref, ref_size, b, 0.1, O.MODE_ALIGNED))
+        for rep in range(3):
+            clear = np.zeros((nc, 2), np.uint32)
+            M.check(M.lib().myersMemcpyHtoD(dr.ptr, M._ptr(clear), clear.nbytes), "clear")
+            M.verify_device(m, dp, ne, dq, nq, dc, nc, 0.1, dr, flags=M.ENGINE_PERSISTENT | M.USE_GRAPH,
+                            h_qinfo=b["qinfo"])
+            M.synchronize(m)
+            np.testing.assert_array_equal(dr.get(), eager, err_msg=f"replay {rep}")
+            m.verify(b["peq"][:8], b["qinfo"][:1], b["cands"][:3], 0.1)     # eager work in between
+        assert m.stats().graphCaptures == 1
+        for x in (dp, dq, dc, dr):
+            x.free()

@@ -16,7 +16,10 @@ the inputs already in HBM.  The script prints ONE JSON line.
 import argparse
 import json
 import os
+import struct
+import subprocess
 import sys
+import tempfile
 import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
@@ -85,9 +88,72 @@ def threshold_of(m, e):
     return int(np.float32(m) * np.float32(e))
 
 
-def cpu_baseline(ref, ref_size, b, gpu_results, seconds):
+REF_PROGRAM = os.path.join(REPO, "oracle", "_ref", "myersCPU-GEM-padding")
+
+
+def reference_cpu(ref, ref_size, b, idx, gpu_results, threads):
+    """Run the reference's own CPU program -- src-integration/myers-reorderBuffer-padding.c,
+    the API's aligned-window twin, compiled from its sources with the reference's
+    flags into oracle/_ref by `make -C oracle ref` -- on the sampled candidates:
+    it reads the .4bits.ref / .bin.qry layouts (myers-reorderBuffer-padding.c:211-272),
+    times its compute (:486-496) and writes "<id> <score> <column>" lines (:404-430).
+    Returns (seconds, None) after checking every result against the GPU's, or
+    (None, reason) when the program cannot run this workload."""
+    if not os.access(REF_PROGRAM, os.X_OK):
+        return None, "oracle/_ref not built (needs the reference sources)"
+    cands = b["cands"][idx]
+    if int(cands["position"].max()) >= 1 << 32:
+        return None, "positions beyond 32 bits (the program's candidates are u32)"
+    if int(b["qinfo"][:, 1].max()) > 4096:
+        return None, "reads over 4096 bases (the program skips them)"
+    qinfo, peq = b["qinfo"], b["peq"]
+    with tempfile.TemporaryDirectory(prefix="myers_ref_") as tmp:
+        ref_path = os.path.join(tmp, "ref.4bits.ref")
+        with open(ref_path, "wb") as f:
+            f.write(struct.pack("<II", len(ref), int(ref_size)))
+            np.ascontiguousarray(ref, dtype="<u4").tofile(f)
+        qry = os.path.join(tmp, "sample.bin.qry")
+        with open(qry, "wb") as f:
+            f.write(struct.pack("<5I", int(qinfo[:, 1].astype(np.int64).sum()), len(peq), 0, len(qinfo), len(idx)))
+            np.ascontiguousarray(peq, dtype="<u4").tofile(f)
+            c8 = np.empty((len(idx), 2), dtype="<u4")
+            c8[:, 0] = cands["query"]
+            c8[:, 1] = cands["position"]
+            c8.tofile(f)
+            np.ascontiguousarray(qinfo, dtype="<u4").tofile(f)
+        # The program's cleanup frees res->h_reorderResults, which initResults
+        # never sets (myers-reorderBuffer-padding.c:278-296,370): a heap chunk
+        # reused from a freed stdio buffer then hands free() file bytes, and the
+        # process dies after writing its results.  glibc's MALLOC_PERTURB_=255
+        # fills each new allocation with 0x00, so that free() gets NULL.
+        run = subprocess.run([REF_PROGRAM, repr(DISTANCE), ref_path, qry], capture_output=True, text=True,
+                             env=dict(os.environ, OMP_NUM_THREADS=str(threads), MALLOC_PERTURB_="255"),
+                             timeout=900)
+        if run.returncode != 0:
+            return None, f"program exited with {run.returncode}: {run.stdout[-200:]} {run.stderr[-200:]}"
+        secs = None
+        for line in run.stdout.splitlines():
+            if line.startswith("TIME COMPUTE MYERS"):
+                secs = float(line.split(":")[1])
+        with open(qry + ".gemres.pad.cpu") as f:
+            tok = f.read().split()
+    if secs is None or int(tok[0]) != len(idx):
+        return None, "unexpected program output"
+    rows = np.array(tok[1:], dtype=np.int64).reshape(-1, 3)       # id, score, column
+    got = np.empty((len(idx), 2), dtype=np.uint32)
+    got[rows[:, 0], 0] = rows[:, 2]
+    got[rows[:, 0], 1] = rows[:, 1]
+    bad = int((got != gpu_results[idx]).any(axis=1).sum())
+    if bad:
+        raise SystemExit(f"bench: GPU results differ from the reference program on {bad} of {len(idx)} candidates")
+    return secs, None
+
+
+def cpu_baseline(ref, ref_size, b, gpu_results, seconds, mode_exact=False):
     """Time the oracle (C restatement of src/myers-4bits*.c, OpenMP) on a
-    seeded sample of the same candidates and check the GPU results on it."""
+    seeded sample of the same candidates and check the GPU results on it;
+    then, where it is built, the reference's own CPU program on the same
+    sample (aligned windows only), which becomes the reported baseline."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_lib as O
     rng = np.random.default_rng(1234)
@@ -117,7 +183,7 @@ def cpu_baseline(ref, ref_size, b, gpu_results, seconds):
     elapsed64 = time.perf_counter() - t0
     if not np.array_equal(res64, res):
         raise SystemExit("bench: the 64-bit BPM baseline differs from the oracle")
-    return {
+    port = {
         "value": checked / elapsed,
         "unit": "candidates/s",
         "gcups": cells / elapsed / 1e9,
@@ -127,9 +193,27 @@ def cpu_baseline(ref, ref_size, b, gpu_results, seconds):
                   f"GPU results bit-exact on all of them",
         "cpu_model": _cpu_model(),
         "algorithm": "u32 blocks with int8 carry (src/myers-4bits.c)",
-        "bpm64": {"value": checked / elapsed64, "unit": "candidates/s", "gcups": cells / elapsed64 / 1e9,
-                  "cores": threads, "kind": "port",
-                  "algorithm": "64-bit block BPM (GEMTools/src/gt_map_align_bpm_generic.c)"},
+    }
+    bpm64 = {"value": checked / elapsed64, "unit": "candidates/s", "gcups": cells / elapsed64 / 1e9,
+             "cores": threads, "kind": "port",
+             "algorithm": "64-bit block BPM (GEMTools/src/gt_map_align_bpm_generic.c)"}
+    secs, why = (None, "exact windows (the program is the aligned-window twin)") if mode_exact else \
+        reference_cpu(ref, ref_size, b, idx, gpu_results, threads)
+    if secs is None:
+        return dict(port, bpm64=bpm64, reference={"skipped": why})
+    return {
+        "value": checked / secs,
+        "unit": "candidates/s",
+        "gcups": cells / secs / 1e9,
+        "cores": threads,
+        "kind": "reference",
+        "sample": f"the same {checked} candidates; the program's own compute time ({secs:.1f} s, OpenMP); "
+                  f"GPU results bit-exact on all of them",
+        "cpu_model": _cpu_model(),
+        "program": "oracle/_ref/myersCPU-GEM-padding: src-integration/myers-reorderBuffer-padding.c "
+                   "built with -O3 -m64 -fopenmp",
+        "port": port,
+        "bpm64": bpm64,
     }
 
 
@@ -380,7 +464,7 @@ def main():
     cpu = None
     if args.cpu_seconds > 0:
         log("CPU baseline / parity sample ...")
-        cpu = cpu_baseline(ref, cfg["ref_size"], b, results, args.cpu_seconds)
+        cpu = cpu_baseline(ref, cfg["ref_size"], b, results, args.cpu_seconds, args.mode == "exact")
 
     serving = None
     if serve_loads:

@@ -511,8 +511,11 @@ typedef struct {
     uint32_t num_present;
     uint32_t single;       /* the variant when num_present == 1 */
     int      too_long;
-    int      sized;        /* reads[] below is filled in */
+    int      sized;        /* reads[] / max_m[] below are filled in */
     uint32_t reads[MYERS_MAX_VARIANTS];   /* reads per variant */
+    uint32_t max_m[MYERS_MAX_VARIANTS];   /* longest read per variant */
+    int      counted;      /* cands[] below is filled in */
+    uint32_t cands[MYERS_MAX_VARIANTS];   /* candidates per variant */
 } bin_scan_t;
 
 static void scan_qinfo(const myers_ctx_t *ctx, const qryInfo_t *qinfo, uint32_t nq, bin_scan_t *s)
@@ -527,6 +530,7 @@ static void scan_qinfo(const myers_ctx_t *ctx, const qryInfo_t *qinfo, uint32_t 
         const uint32_t v = ctx->bin_of_words[w];
         s->present |= 1ull << v;
         s->reads[v]++;
+        if (m > s->max_m[v]) s->max_m[v] = m;
     }
     for (uint32_t v = 0; v < ctx->num_variants; v++)
         if (s->present & (1ull << v)) { s->num_present++; s->single = v; }
@@ -619,14 +623,27 @@ static uint32_t present_by_cost(const myers_ctx_t *ctx, uint64_t present, uint32
 
 /* Long-read bins of a persistent batch that go into one launch
  * (myers_launch_long): bins of 4..64 lanes per candidate, longest reads
- * first, when their chunks (estimated from the reads per bin and the batch's
- * candidates per read) fit a few rounds of one wave per SIMD.  Such bins are
+ * first, when their chunks (counted from the host's candidates, or estimated
+ * from the reads per bin and the batch's candidates per read) fit a few
+ * rounds of one wave per SIMD.  Such bins are
  * bound by each candidate's chain of column steps; larger long-read batches
- * are throughput work and keep one launch per bin.  While one round of waves
- * still holds everything, the longest bins (over 256 words) run each
- * candidate on a whole 4-wave workgroup, which shortens its chain.  Returns
- * the bins taken (0: none). */
+ * are throughput work and keep one launch per bin.
+ *
+ * A bin of reads over 256 words can instead run each candidate on a whole
+ * 4-wave workgroup, which shortens its chain (measured: 20,000 bases in 3.2 ms
+ * against 5.9 ms on one wave) for 4 SIMDs instead of one.  The plan widens
+ * the bin with the longest estimated chain while that lowers the estimated
+ * time of the launch, max(longest chain, SIMD-time of all items / SIMDs).
+ * Per column step, from single-candidate runs: one wave of W words per lane
+ * ~40 + 20 W ns, a workgroup of W words per lane ~95 + 12 W ns.  Returns the
+ * bins taken (0: none). */
 #define LONG_ROUNDS 4u
+static double chain_ns(uint32_t max_m, uint32_t lanes, uint32_t words, int wide)
+{
+    const double steps = 1.2 * (double)max_m + (double)lanes;
+    return steps * (wide ? 95.0 + 12.0 * (double)words : 40.0 + 20.0 * (double)words);
+}
+
 static uint32_t long_plan(const myers_ctx_t *ctx, const bin_scan_t *scan, const batch_t *bt,
                           const uint32_t *order, uint32_t n, myers_long_plan_t *lp)
 {
@@ -634,31 +651,52 @@ static uint32_t long_plan(const myers_ctx_t *ctx, const bin_scan_t *scan, const 
     if (!scan->sized || bt->num_queries == 0) return 0;
     const double cpr = (double)bt->num_cands / (double)bt->num_queries;
     const double simds = (double)MYERS_LONG_WAVES * (double)ctx->num_cus;
+    const uint32_t wide_lanes = 64u * MYERS_LONG_WAVES;
     uint32_t take[MYERS_LONG_MAX], nt = 0;
-    double waves = 0.0;
+    double waves[MYERS_LONG_MAX], chain[MYERS_LONG_MAX], total_waves = 0.0, work = 0.0;
+    int wide[MYERS_LONG_MAX] = {0};
     for (uint32_t i = 0; i < n && nt < MYERS_LONG_MAX; i++) {
         const uint32_t v = order[i];
         if (!myers_variant_long_ok(v, NULL)) continue;
+        const myers_variant_desc_t *d = &ctx->variants[v];
+        const double cands = scan->counted ? (double)scan->cands[v] : (double)scan->reads[v] * cpr;
+        waves[nt] = cands / (double)d->slots;
+        chain[nt] = chain_ns(scan->max_m[v], d->lanes, d->words, 0);
+        total_waves += waves[nt];
+        work += waves[nt] * chain[nt];
         take[nt++] = v;
-        waves += (double)scan->reads[v] * cpr / (double)ctx->variants[v].slots;
     }
-    if (nt == 0 || waves > (double)LONG_ROUNDS * simds) return 0;
-    /* order[] is longest first: widen bins while one round still fits */
-    uint32_t nw = 0;
-    for (; nw < nt; nw++) {
-        const uint32_t v = take[nw];
-        const uint32_t words = ctx->variants[v].total_words;
-        if (words <= 64u * MYERS_LONG_WAVES || words > 64u * MYERS_LONG_WAVES * MYERS_WIDE_MAX_WORDS) break;
-        const double cands = (double)scan->reads[v] * cpr;
-        const double more = cands * (double)MYERS_LONG_WAVES - cands / (double)ctx->variants[v].slots;
-        if (waves + more > simds) break;
-        waves += more;
-        lp->wide_variant[nw] = v;
-        lp->wide_words[nw] = (words + 64u * MYERS_LONG_WAVES - 1u) / (64u * MYERS_LONG_WAVES);
+    if (nt == 0 || total_waves > (double)LONG_ROUNDS * simds) return 0;
+    for (;;) {
+        uint32_t top = 0;
+        double longest = 0.0;
+        for (uint32_t k = 0; k < nt; k++)
+            if (chain[k] > longest) { longest = chain[k]; top = k; }
+        const myers_variant_desc_t *d = &ctx->variants[take[top]];
+        if (wide[top] || d->total_words <= wide_lanes || d->total_words > wide_lanes * MYERS_WIDE_MAX_WORDS) break;
+        const uint32_t ww = (d->total_words + wide_lanes - 1u) / wide_lanes;
+        const double cands = waves[top] * (double)d->slots;
+        const double c_wide = chain_ns(scan->max_m[take[top]], wide_lanes, ww, 1);
+        const double work_wide = work - waves[top] * chain[top] + (double)MYERS_LONG_WAVES * cands * c_wide;
+        double next = c_wide;
+        for (uint32_t k = 0; k < nt; k++)
+            if (k != top && chain[k] > next) next = chain[k];
+        const double before = longest > work / simds ? longest : work / simds;
+        const double after = next > work_wide / simds ? next : work_wide / simds;
+        if (after >= before) break;
+        wide[top] = 1;
+        chain[top] = c_wide;
+        work = work_wide;
     }
-    lp->n_wide = nw;
-    for (uint32_t k = nw; k < nt; k++) {
+    for (uint32_t k = 0; k < nt; k++) {            /* wide bins keep the longest-first order */
         const uint32_t v = take[k];
+        const uint32_t words = ctx->variants[v].total_words;
+        if (wide[k]) {
+            lp->wide_variant[lp->n_wide] = v;
+            lp->wide_words[lp->n_wide] = (words + wide_lanes - 1u) / wide_lanes;
+            lp->n_wide++;
+            continue;
+        }
         uint32_t u4 = 0;
         myers_variant_long_ok(v, &u4);
         lp->variant[lp->n] = v;
@@ -666,7 +704,7 @@ static uint32_t long_plan(const myers_ctx_t *ctx, const bin_scan_t *scan, const 
         lp->n++;
         if (u4 > lp->ws_u4) lp->ws_u4 = u4;
     }
-    if (nw) {
+    if (lp->n_wide) {
         const uint32_t u4 = (myers_long_wide_lds_bytes() + 16u * MYERS_LONG_WAVES - 1u) / (16u * MYERS_LONG_WAVES);
         if (u4 > lp->ws_u4) lp->ws_u4 = u4;
     }
@@ -779,10 +817,32 @@ static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan
     return MYERS_SUCCESS;
 }
 
-static void make_scan(myers_ctx_t *ctx, const qryInfo_t *h_qinfo, uint32_t num_queries, bin_scan_t *scan)
+/* Candidates per variant, for batches with long-read bins (the long-read
+ * plan sizes its rounds with them; reads carry very different candidate
+ * counts across lengths).  Short-read batches skip this pass. */
+static void scan_cands(const myers_ctx_t *ctx, const qryInfo_t *qinfo, uint32_t nq, const candInfo_t *cands,
+                       uint32_t nc, bin_scan_t *s)
+{
+    uint64_t long_mask = 0;
+    for (uint32_t v = 0; v < ctx->num_variants; v++)
+        if (myers_variant_long_ok(v, NULL)) long_mask |= 1ull << v;
+    if (!(s->present & long_mask)) return;
+    const uint32_t maxw = max_supported_words(ctx);
+    for (uint32_t i = 0; i < nc; i++) {
+        const uint32_t q = cands[i].query;
+        if (q >= nq) continue;
+        const uint32_t w = (qinfo[q].size + 31u) >> 5;
+        if (w <= maxw) s->cands[ctx->bin_of_words[w]]++;
+    }
+    s->counted = 1;
+}
+
+static void make_scan(myers_ctx_t *ctx, const qryInfo_t *h_qinfo, uint32_t num_queries, const candInfo_t *h_cands,
+                      uint32_t num_cands, bin_scan_t *scan)
 {
     if (h_qinfo) {
         scan_qinfo(ctx, h_qinfo, num_queries, scan);
+        if (h_cands) scan_cands(ctx, h_qinfo, num_queries, h_cands, num_cands, scan);
     } else {
         memset(scan, 0, sizeof(*scan));
         scan->present = ctx->num_variants >= 64 ? ~0ull : ((1ull << ctx->num_variants) - 1u);
@@ -794,10 +854,10 @@ static void make_scan(myers_ctx_t *ctx, const qryInfo_t *h_qinfo, uint32_t num_q
 /* Enqueue the verification of a batch on b->stream, timed with HIP events;
  * with MYERS_USE_GRAPH the pipeline is captured once per batch shape and
  * replayed as a graph. */
-static int enqueue_verify(myers_buffer_t *b, const batch_t *bt, const qryInfo_t *h_qinfo)
+static int enqueue_verify(myers_buffer_t *b, const batch_t *bt, const qryInfo_t *h_qinfo, const candInfo_t *h_cands)
 {
     bin_scan_t scan;
-    make_scan(b->ctx, h_qinfo, bt->num_queries, &scan);
+    make_scan(b->ctx, h_qinfo, bt->num_queries, h_cands, bt->num_cands, &scan);
     uint32_t slot = 0;
     int rc;
     if (!(bt->flags & MYERS_USE_GRAPH) || bt->num_cands == 0) {
@@ -888,7 +948,7 @@ int myersSendEx(void *myersBuffer, float distance, uint32_t numPEQEntries, uint3
         SEND_TRY(hipMemcpyAsync(b->d_cands, b->h_cands, (size_t)numCandidates * sizeof(candInfo_t), hipMemcpyHostToDevice, s));
     const batch_t bt = {b->d_peq, b->d_qinfo, b->d_cands, b->d_results, numPEQEntries, numQueries,
                         numCandidates, flags, distance, b->threshold};
-    rc = enqueue_verify(b, &bt, b->h_qinfo);
+    rc = enqueue_verify(b, &bt, b->h_qinfo, b->h_cands);
     if (rc) return (b->last_error = rc);
     if (numCandidates)
         SEND_TRY(hipMemcpyAsync(b->h_results, b->d_results, (size_t)numCandidates * sizeof(resEntry_t), hipMemcpyDeviceToHost, s));
@@ -969,7 +1029,7 @@ int myersVerifyDevice(void *myersBuffer, const void *d_peq, uint32_t numPEQEntri
     *b->h_err = 0;
     const batch_t bt = {d_peq, d_qinfo, d_candidates, d_results, numPEQEntries, numQueries, numCandidates,
                         flags, distance, b->threshold};
-    rc = enqueue_verify(b, &bt, h_qinfo);
+    rc = enqueue_verify(b, &bt, h_qinfo, NULL);
     b->stats.cells = 0;
     b->stats.candidates = numCandidates;
     b->pending = (rc == MYERS_SUCCESS);

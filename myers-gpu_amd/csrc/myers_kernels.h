@@ -72,11 +72,12 @@ int myers_variant_occupancy_persistent(uint32_t v, int *wg_per_cu);
 /* Long-read bins in one launch.  Bins of one-wave variants with 4..64 lanes
  * per candidate (myers_variant_long_ok) can run together in workgroups of
  * MYERS_LONG_WAVES waves: first the plan's wide bins, each candidate on a
- * whole workgroup, then the others, each wave on its own.  Items (chunks of the plan's bins, in
- * plan order) 0 .. waves-1 go one to a wave, starting at wave `rotate`, so
- * that successive launches start their longest items on other workgroups;
- * later items are pulled from one counter (queue_heads + MYERS_LONG_HEAD,
- * reset by the binning).  Needs bin_ranges and queue_heads. */
+ * whole workgroup, then the others (chunks of the plan's bins, in plan
+ * order), each wave on its own.  The first round of each goes one item to a
+ * workgroup / wave, from `rotate` on, so that successive launches start their
+ * longest items on other CUs; later items are pulled from one counter each
+ * (queue_heads + MYERS_WIDE_HEAD / MYERS_LONG_HEAD, reset by the binning).
+ * Needs bin_ranges and queue_heads. */
 enum { MYERS_LONG_MAX = 16, MYERS_LONG_WAVES = 4 };
 #define MYERS_LONG_HEAD ((MYERS_MAX_VARIANTS - 1) * MYERS_QUEUE_PARTS * MYERS_QUEUE_STRIDE)
 #define MYERS_WIDE_HEAD (MYERS_LONG_HEAD + MYERS_QUEUE_STRIDE)
@@ -88,7 +89,10 @@ uint32_t myers_long_wide_lds_bytes(void);
 typedef struct {
     uint32_t n;                          /* one-wave bins, in pull order */
     uint32_t ws_u4;                      /* LDS per wave in 16-byte units (max over the bins) */
-    uint32_t rotate;                     /* first-round item of wave s is (s - rotate) mod waves */
+    uint32_t rotate;                     /* first round starts at wave `rotate` (a multiple of
+                                            MYERS_LONG_WAVES): wide items on workgroups from
+                                            rotate / MYERS_LONG_WAVES on, then one-wave items on
+                                            the waves after them */
     uint32_t variant[MYERS_LONG_MAX];    /* variant ids (rows of bin_ranges) */
     uint32_t key[MYERS_LONG_MAX];        /* lanes << 8 | words of each */
     /* bins run first, one candidate per workgroup (MYERS_LONG_WAVES waves of

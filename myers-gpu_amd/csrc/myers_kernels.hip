@@ -962,19 +962,27 @@ __global__ __launch_bounds__(64 * MYERS_LONG_WAVES) void verify_long_kernel(myer
         }
         __syncthreads();                         // phase 2 reuses the LDS
     }
-    // Phase 2: the other bins, one chunk per wave.
+    // Phase 2: the other bins, one chunk per wave.  The first round gives one
+    // item to each wave free at the start -- past the workgroups that took a
+    // wide candidate in phase 1's first round (from workgroup rotate / waves
+    // on) -- so that the longest of these items start at once; later items
+    // are pulled from the counter.
     uint4 *ws = long_lds + wave * p.ws_u4;
     uint32_t *counter = a.queue_heads + MYERS_LONG_HEAD;
     uint32_t prepared = 0xFFFFFFFFu;
-    // first round: one item per wave, placed from wave `rotate` on; then pulls
+    uint32_t wide_first = 0;
+    for (uint32_t j = 0; j < p.n_wide; j++) wide_first += a.bin_ranges[2u * p.wide_variant[j] + 1u];
+    wide_first = min(wide_first, gridDim.x);
     const uint32_t nw = gridDim.x * (uint32_t)MYERS_LONG_WAVES;
+    const uint32_t free_waves = nw - wide_first * (uint32_t)MYERS_LONG_WAVES;
     const uint32_t slot = blockIdx.x * (uint32_t)MYERS_LONG_WAVES + wave;
-    uint32_t item = (slot + nw - p.rotate % nw) % nw;
-    for (bool first = true;; first = false) {
+    const uint32_t rel = (slot + nw - (p.rotate + wide_first * (uint32_t)MYERS_LONG_WAVES) % nw) % nw;
+    uint32_t item = rel;
+    for (bool first = rel < free_waves;; first = false) {
         if (!first) {
             uint32_t got = 0;
             if (lane == 0u) got = atomicAdd(counter, 1u);
-            item = nw + (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)got, 0));
+            item = free_waves + (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)got, 0));
         }
         uint32_t j = 0, base = 0, start = 0, count = 0;
         bool found = false;

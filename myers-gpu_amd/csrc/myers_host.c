@@ -661,7 +661,7 @@ static uint32_t long_plan(const myers_ctx_t *ctx, const bin_scan_t *scan, const 
         const myers_variant_desc_t *d = &ctx->variants[v];
         const double cands = scan->counted ? (double)scan->cands[v] : (double)scan->reads[v] * cpr;
         waves[nt] = cands / (double)d->slots;
-        chain[nt] = chain_ns(scan->max_m[v], d->lanes, d->words, 0);
+        chain[nt] = cands > 0.0 ? chain_ns(scan->max_m[v], d->lanes, d->words, 0) : 0.0;
         total_waves += waves[nt];
         work += waves[nt] * chain[nt];
         take[nt++] = v;
@@ -673,7 +673,9 @@ static uint32_t long_plan(const myers_ctx_t *ctx, const bin_scan_t *scan, const 
         for (uint32_t k = 0; k < nt; k++)
             if (chain[k] > longest) { longest = chain[k]; top = k; }
         const myers_variant_desc_t *d = &ctx->variants[take[top]];
-        if (wide[top] || d->total_words <= wide_lanes || d->total_words > wide_lanes * MYERS_WIDE_MAX_WORDS) break;
+        if (longest <= 0.0 || wide[top] || d->total_words <= wide_lanes ||
+            d->total_words > wide_lanes * MYERS_WIDE_MAX_WORDS)
+            break;
         const uint32_t ww = (d->total_words + wide_lanes - 1u) / wide_lanes;
         const double cands = waves[top] * (double)d->slots;
         const double c_wide = chain_ns(scan->max_m[take[top]], wide_lanes, ww, 1);

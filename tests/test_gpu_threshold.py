@@ -120,3 +120,21 @@ def test_threshold_graph_recaptures_on_change():
         assert m.stats().graphCaptures == 3
         for x in (dp, dq, dc, dr):
             x.free()
+
+
+def test_threshold_long_read_launch():
+    """Threshold mode through the long-read launch: reads over 8 kb run on whole
+    workgroups, 3-5 kb reads on one wave, next to short-read bins."""
+    from test_gpu_parity import concat_batches
+    size = 3_000_000
+    ref = M.synth_reference(67, size)
+    b = concat_batches([M.synth_batch(M.make_spec(68 + k, n, length=length, sub=0.04, ins=0.02, dele=0.02,
+                                                  candidates=c, distance=0.1), ref, size)
+                        for k, (n, length, c) in enumerate([(150, 200, 6), (5, 3500, 3), (4, 9000, 3),
+                                                            (3, 17000, 2)])])
+    with M.Myers(ref, size, avg_query=600, cands_per_query=5, buffer_bytes=256 << 20) as m:
+        for e in (0.05, 0.15, 0.3):
+            announce(f"threshold long-read launch e={e}")
+            m.set_threshold(e)
+            got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=M.ENGINE_PERSISTENT)
+            np.testing.assert_array_equal(got, expected(ref, size, b, 0.1, 0, e))

@@ -156,6 +156,7 @@ def cpu_baseline(ref, ref_size, b, gpu_results, seconds, mode_exact=False):
     sample (aligned windows only), which becomes the reported baseline."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_lib as O
+    omode = O.MODE_EXACT if mode_exact else O.MODE_ALIGNED
     rng = np.random.default_rng(1234)
     cands = b["cands"]
     order = rng.permutation(len(cands))
@@ -165,7 +166,7 @@ def cpu_baseline(ref, ref_size, b, gpu_results, seconds, mode_exact=False):
     while True:
         idx = np.sort(order[:n])
         t0 = time.perf_counter()
-        res = O.myers_batch(ref, ref_size, b["peq"], b["qinfo"], cands[idx], DISTANCE, O.MODE_ALIGNED)
+        res = O.myers_batch(ref, ref_size, b["peq"], b["qinfo"], cands[idx], DISTANCE, omode)
         elapsed = time.perf_counter() - t0
         mismatches = int((res != gpu_results[idx]).any(axis=1).sum())
         checked = n
@@ -179,7 +180,7 @@ def cpu_baseline(ref, ref_size, b, gpu_results, seconds, mode_exact=False):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     # GEMTools' block BPM over 64-bit words on the same sample (a second CPU baseline)
     t0 = time.perf_counter()
-    res64 = O.bpm64_batch(ref, ref_size, b["peq"], b["qinfo"], cands[idx], DISTANCE, O.MODE_ALIGNED)
+    res64 = O.bpm64_batch(ref, ref_size, b["peq"], b["qinfo"], cands[idx], DISTANCE, omode)
     elapsed64 = time.perf_counter() - t0
     if not np.array_equal(res64, res):
         raise SystemExit("bench: the 64-bit BPM baseline differs from the oracle")

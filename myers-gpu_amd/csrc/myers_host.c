@@ -13,9 +13,11 @@
  *            scatter (myersGPU.c:500-513) do not exist here.
  *   launch:  the host scans qinfo (O(queries)) for the length bins present.
  *            One bin -> one launch over the candidates as they are.  More
- *            bins -> device-side binning (count, scan, scatter) and one
- *            launch per present bin (plain engine), or a single
- *            persistent launch (persistent engine, see below).
+ *            bins -> device-side binning (count, scan, base, scatter), then
+ *            one launch per present bin in turn (plain engine), or one
+ *            resident launch per bin on forked streams (persistent engine),
+ *            where the long-read bins of a latency-bound batch share one
+ *            launch (long_plan, myers_launch_long).
  *   receive: hipStreamSynchronize; results are in the pinned results buffer.
  *
  * All buffers of a library instance live on one device and share one copy

@@ -251,12 +251,15 @@ __device__ __forceinline__ void zero_row_init(Workspace<L, W> &ws) {
 
 // Verify chunks chunk_lo, chunk_lo + stride, ... < chunk_hi of the slot range
 // [start, start + count); a chunk is the kSlots candidates of one wave.
-// kCut (L = 1, W % 4 == 0): threshold mode with the Ukkonen cut-off.
+// kCut: threshold mode.  L = 1 (W % 4 == 0): the Ukkonen cut-off; L > 1: a
+// wave stops once its candidates cannot get under the threshold.
 template <int L, int W, bool kCut, int kWaves = 1>
 __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint32_t start, uint32_t count,
                                               uint32_t chunk_lo, uint32_t chunk_hi, uint32_t stride,
                                               Workspace<L, W> &ws) {
     using S = Shape<L, W>;
+    constexpr bool kUkk = kCut && L == 1;       // the cut-off
+    constexpr bool kStop = kCut && L > 1;       // the early stop
     uint4 *lds = ws.lds;
     uint2 *qtab = ws.qtab;
     const uint32_t lane = wave_lane<kWaves>();
@@ -326,6 +329,12 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
             chunk_sync<kWaves>();
             const uint4 *lds_lane = lds + my_slot * (uint32_t)S::kSlotU4 + gl;
             const uint32_t ncols = in_pass ? ncols_live : 0u;
+            // threshold of the group's top lane (-1: no early stop for this lane)
+            int32_t kthr = -1;
+            if constexpr (kStop) {
+                if (a.threshold >= 0.0f && live && in_pass && gl == (uint32_t)(L - 1))
+                    kthr = (int32_t)threshold_of(m, a.threshold);
+            }
 
             // Steps: the top lane of a group finishes column ncols-1 at step
             // ncols-1 + L-1.  Lane gl reads text from nibble nib0 - gl.
@@ -351,7 +360,7 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
             constexpr uint32_t kG = (uint32_t)(W / 4);
             int32_t kv = 0;
             uint32_t ga = kG;
-            if constexpr (kCut) {
+            if constexpr (kUkk) {
                 kv = (int32_t)threshold_of(m, a.threshold) + (32 * W - (int32_t)m);
                 ga = min(wave_max(ncols > 0u ? ((uint32_t)kv + 135u) / 128u : 1u), kG);
                 score = 128 * (int32_t)ga;
@@ -385,7 +394,7 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
             uint4 eq4 = lds_lane[cur_off];
             for (uint32_t b = 0; b < nblocks; b++) {
                 const uint32_t tw4 = text_word((int32_t)b + 4);
-                if constexpr (kCut) {
+                if constexpr (kUkk) {
                     // The extent changes only between blocks of 8 columns.  A row
                     // with D <= k in column j + t lies at most t + 1 rows above
                     // the last such row of column j - 1, so the extent must keep
@@ -422,20 +431,20 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
                     }
                 }
                 uint32_t ga_v = ga;
-                if constexpr (kCut) asm volatile("v_mov_b32 %0, %1" : "=v"(ga_v) : "s"(ga));
+                if constexpr (kUkk) asm volatile("v_mov_b32 %0, %1" : "=v"(ga_v) : "s"(ga));
 #pragma unroll
                 for (int i = 0; i < 8; i++) {
                     const uint32_t cn = (i < 7) ? ((text >> (4 * (i + 1))) & 7u) : (text_next & 7u);
                     const uint32_t next_off = cn * S::kRowU4;
                     if constexpr (L == 1) {
                         const uint4 eq4n = lds_lane[next_off];        // next column, group 0
-                        column_grp<W, false, kCut, (uint32_t)S::kGroupU4>(
+                        column_grp<W, false, kUkk, (uint32_t)S::kGroupU4>(
                             Pv, Mv, eq4, lds_lane, cur_off, ga_v, 0u, 0u, ph_top, mh_top);
                         eq4 = eq4n;
                         score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
                         const uint32_t col = 8u * b + (uint32_t)i;
                         const bool on = col < ncols;
-                        if constexpr (!kCut) {
+                        if constexpr (!kUkk) {
                             // first minimum of the score row (src-api/coreMyersGPU-CC35.cu:205-207,227-228)
                             const bool better = (score < min_score) && on;
                             min_col = better ? col : min_col;
@@ -472,6 +481,19 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
                 text_next = text_block(tw2, tw3, b + 2);
                 tw2 = tw3;
                 tw3 = tw4;
+                if constexpr (kStop) {
+                    // Threshold mode: the score row moves by at most 1 per column, so a
+                    // candidate whose minimum so far and whose score less the columns
+                    // left both exceed the threshold cannot get under it.  The wave
+                    // stops once each of its candidates has finished or is in that state.
+                    if (a.threshold >= 0.0f) {
+                        const int32_t col = (int32_t)(8u * b + 7u) - (L - 1);
+                        const bool hopeless = col >= 0 && min_score > kthr &&
+                                              score - ((int32_t)ncols - 1 - col) > kthr;
+                        const bool running = kthr >= 0 && col < (int32_t)ncols - 1 && !hopeless;
+                        if (__ballot(running) == 0ull) break;
+                    }
+                }
             }
 
             if (in_pass && valid && gl == (uint32_t)(L - 1)) {
@@ -1177,7 +1199,7 @@ struct VariantEntry {
     uint32_t block;                  // threads per workgroup
 };
 template <int L, int W>
-constexpr bool kHasCut = L == 1 && W >= 8 && W % 4 == 0;
+constexpr bool kHasCut = (L == 1 && W >= 8 && W % 4 == 0) || (L > 1 && L <= 64);
 template <int L, int W>
 constexpr const void *cut_kernel(bool persistent) {
     if constexpr (kHasCut<L, W>) {

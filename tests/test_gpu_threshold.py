@@ -138,3 +138,21 @@ def test_threshold_long_read_launch():
             m.set_threshold(e)
             got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=M.ENGINE_PERSISTENT)
             np.testing.assert_array_equal(got, expected(ref, size, b, 0.1, 0, e))
+
+
+@pytest.mark.parametrize("mode", [M.WINDOW_ALIGNED, M.WINDOW_EXACT])
+def test_threshold_long_reads_early_stop(mode):
+    """Long noisy reads (1 true and 3 random candidates each) on the per-bin
+    multi-lane kernels in threshold mode, where a wave stops once its
+    candidates cannot get under the threshold; rates on both sides of the
+    true loci's error."""
+    size = 3_000_000
+    ref = M.synth_reference(72, size)
+    b = M.synth_batch(M.make_spec(73, 24, log_uniform=(1500, 20000), err_range=(0.12, 0.12), sub_frac=0.5,
+                                  candidates=4, distance=0.1), ref, size)
+    with M.Myers(ref, size, avg_query=8000, cands_per_query=4, buffer_bytes=256 << 20) as m:
+        for e in (0.05, 0.15, 0.25):
+            announce(f"threshold long reads early stop mode={mode} e={e}")
+            m.set_threshold(e)
+            got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=mode | M.ENGINE_PLAIN)
+            np.testing.assert_array_equal(got, expected(ref, size, b, 0.1, mode, e))

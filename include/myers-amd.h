@@ -113,8 +113,10 @@ int myersReceiveEx(void *myersBuffer);
  * threshold, and every other verified candidate as
  * {column 0xFFFFFFFF, score 0xFFFFFFFF}; skipped candidates keep
  * {0, 0xFFFFFFFF}.  Kernels for queries of up to 1024 bases then skip the
- * rows whose distance already exceeds the threshold (Ukkonen cut-off).
- * errorRate < 0 switches the mode off (the default). */
+ * rows whose distance already exceeds the threshold (Ukkonen cut-off), and
+ * the multi-lane kernels of per-bin launches stop a wave once none of its
+ * candidates can get under the threshold.  errorRate < 0 switches the mode
+ * off (the default). */
 #define MYERS_NO_MATCH 0xFFFFFFFFu
 int   myersSetThreshold(void *myersBuffer, float errorRate);
 float myersGetThreshold(void *myersBuffer);

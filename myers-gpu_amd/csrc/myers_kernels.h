@@ -47,7 +47,8 @@ typedef struct {
     uint32_t        mode;        /* 0 aligned window, 1 exact-start window */
     float           distance;
     float           threshold;   /* < 0: off; else results above (uint32)(m*threshold) get
-                                    {0xFFFFFFFF, 0xFFFFFFFF} and L = 1 variants cut off rows */
+                                    {0xFFFFFFFF, 0xFFFFFFFF}; L = 1 variants cut off rows and
+                                    L = 4..64 variants stop hopeless waves */
     uint32_t        chunk_factor;  /* persistent launch: chunks per queue pull */
     void           *results;     /* resEntry_t[] in caller order */
     uint32_t       *queue_heads; /* persistent launch: MYERS_QUEUE_WORDS words of part heads */

@@ -79,8 +79,8 @@ def test_threshold_config2_shape(mode):
 
 
 def test_threshold_mixed_lengths_and_long_reads():
-    """Mixed short reads and long reads (multi-lane kernels post-filter) in
-    one batch, with skip-rule candidates at the reference end."""
+    """Mixed short reads and long reads (multi-lane kernels) in one batch,
+    with skip-rule candidates at the reference end."""
     size = 3_000_000
     ref = M.synth_reference(64, size)
     b = M.synth_batch(M.make_spec(65, 900, choices=[50, 100, 200, 333, 600, 1000, 1500, 3000],

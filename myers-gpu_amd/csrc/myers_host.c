@@ -632,18 +632,18 @@ static uint32_t present_by_cost(const myers_ctx_t *ctx, uint64_t present, uint32
  * are throughput work and keep one launch per bin.
  *
  * A bin of reads over 256 words can instead run each candidate on a whole
- * 4-wave workgroup, which shortens its chain (measured: 20,000 bases in 3.2 ms
+ * 4-wave workgroup, which shortens its chain (measured: 20,000 bases in 2.9 ms
  * against 5.9 ms on one wave) for 4 SIMDs instead of one.  The plan widens
  * the bin with the longest estimated chain while that lowers the estimated
  * time of the launch, max(longest chain, SIMD-time of all items / SIMDs).
  * Per column step, from single-candidate runs: one wave of W words per lane
- * ~40 + 20 W ns, a workgroup of W words per lane ~95 + 12 W ns.  Returns the
+ * ~40 + 20 W ns, a workgroup of W words per lane ~63 + 18.5 W ns.  Returns the
  * bins taken (0: none). */
 #define LONG_ROUNDS 4u
 static double chain_ns(uint32_t max_m, uint32_t lanes, uint32_t words, int wide)
 {
     const double steps = 1.2 * (double)max_m + (double)lanes;
-    return steps * (wide ? 95.0 + 12.0 * (double)words : 40.0 + 20.0 * (double)words);
+    return steps * (wide ? 63.0 + 18.5 * (double)words : 40.0 + 20.0 * (double)words);
 }
 
 static uint32_t long_plan(const myers_ctx_t *ctx, const bin_scan_t *scan, const batch_t *bt,

@@ -39,6 +39,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "myers_kernels.h"
 
 namespace {
@@ -703,8 +705,16 @@ __device__ __forceinline__ uint2 wide_columns(WideWorkspace<Q, W> &ws, uint32_t 
         Pv[k] = ~0u;
         Mv[k] = 0u;
     }
+    // Score row and its first minimum.  Before its first column the top lane
+    // keeps its initial state (every lane below it is before its own first
+    // column too), so the score stays m and those steps never win; only the
+    // last block can reach columns past the window and checks them.  Windows
+    // of at most 65,536 columns keep the minimum as one key, (score << 16) |
+    // column (score <= m <= 24,576): the smaller key is the smaller score, then
+    // the earlier column; a column past the window is masked to all ones.
     int32_t score = (int32_t)m, min_score = (int32_t)m;
-    uint32_t min_col = 0, ph_top = 0, mh_top = 0, acc_p = 0, acc_m = 0;
+    uint32_t min_col = 0, min_key = m << 16;
+    uint32_t ph_top = 0, mh_top = 0, acc_p = 0, acc_m = 0;
     auto text_word = [&](int32_t k) { return ref[min(max(w0 + k, 0), last_word)]; };
     auto text_block = [&](uint32_t lo, uint32_t hi, uint32_t b) {
         uint32_t t = sanitize_text(__builtin_amdgcn_alignbit(hi, lo, shift));
@@ -724,7 +734,9 @@ __device__ __forceinline__ uint2 wide_columns(WideWorkspace<Q, W> &ws, uint32_t 
     uint32_t text_next = text_block(tw1, tw2, 1);
     uint32_t cur_off = (text & 7u) * (uint32_t)S::kRowU4;
     uint4 eq4 = lds_lane[cur_off];
-    for (uint32_t b = 0; b < nblocks; b++) {
+    auto block = [&](uint32_t b, auto last, auto keyed) {
+        constexpr bool kLast = decltype(last)::value;
+        constexpr bool kKeyed = decltype(keyed)::value;
         const uint32_t tw4 = text_word((int32_t)b + 4);
         // ring words of the wave below: its top lane's steps 8b-9 .. 8b-2,
         // written in blocks b-2 and b-1 (the same value in every lane)
@@ -758,9 +770,14 @@ __device__ __forceinline__ uint2 wide_columns(WideWorkspace<Q, W> &ws, uint32_t 
                 // score row: the top lane of the top wave, at column t - kMaxLag
                 score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
                 const uint32_t col = 8u * b + (uint32_t)i - S::kMaxLag;
-                const bool better = (score < min_score) && (col < ncols);
-                min_col = better ? col : min_col;
-                min_score = better ? score : min_score;
+                if constexpr (kKeyed) {
+                    const uint32_t c = kLast ? (col < ncols ? col : ~0u) : col;
+                    min_key = min(min_key, ((uint32_t)score << 16) | c);
+                } else {
+                    const bool better = kLast ? (score < min_score) && (col < ncols) : (score < min_score);
+                    min_col = better ? col : min_col;
+                    min_score = better ? score : min_score;
+                }
             }
             cur_off = next_off;
             __builtin_amdgcn_sched_barrier(0);
@@ -773,7 +790,16 @@ __device__ __forceinline__ uint2 wide_columns(WideWorkspace<Q, W> &ws, uint32_t 
         text_next = text_block(tw2, tw3, b + 2);
         tw2 = tw3;
         tw3 = tw4;
+    };
+    auto run = [&](auto keyed) {
+        for (uint32_t b = 0; b + 1u < nblocks; b++) block(b, std::false_type{}, keyed);
+        if (nblocks) block(nblocks - 1u, std::true_type{}, keyed);
+    };
+    if (kTop && ncols <= 0x10000u) {
+        run(std::true_type{});
+        return make_uint2(min_key & 0xFFFFu, min_key >> 16);
     }
+    run(std::false_type{});
     return make_uint2(min_col, (uint32_t)min_score);
 }
 

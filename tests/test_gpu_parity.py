@@ -403,3 +403,24 @@ def test_graph_replays_long_read_launch():
         assert m.stats().graphCaptures == 1
         for x in (dp, dq, dc, dr):
             x.free()
+
+
+@pytest.mark.parametrize("mode", [M.WINDOW_ALIGNED, M.WINDOW_EXACT])
+def test_multiwave_windows_over_65536_columns(mode):
+    """Distance 1.0 gives 20-24 kb reads windows of 60,000-73,700 columns: the
+    multi-wave top wave keeps its first minimum as (score, column) pairs there
+    instead of one 16-bit-column key."""
+    announce(f"multi-wave wide windows mode={mode}")
+    ref_size = 2_000_000
+    ref = M.synth_reference(44, ref_size)
+    b = M.synth_batch(M.make_spec(45, 3, choices=[20000, 22000, 24500], sub=0.05, ins=0.02, dele=0.02,
+                                  candidates=2, distance=1.0), ref, ref_size)
+    with M.Myers(ref, ref_size, avg_query=22000, cands_per_query=2, buffer_bytes=256 << 20) as m:
+        variants = m.variants()
+        wide = {(v["lanes"], v["words"]): i for i, v in enumerate(variants) if v["lanes"] > 64}
+        table = m.bin_table()
+        w = np.arange(len(table))
+        table[w > 512] = wide[(256, 3)]
+        m.set_bin_table(table)
+        got = m.verify(b["peq"], b["qinfo"], b["cands"], 1.0, flags=mode | M.ENGINE_PLAIN)
+        np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b, 1.0, mode))

@@ -424,3 +424,35 @@ def test_multiwave_windows_over_65536_columns(mode):
         m.set_bin_table(table)
         got = m.verify(b["peq"], b["qinfo"], b["cands"], 1.0, flags=mode | M.ENGINE_PLAIN)
         np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b, 1.0, mode))
+
+
+def test_long_read_launch_edge_cases():
+    """Long-read bins in the one launch with nothing to do: reads without
+    candidates, candidates that all take the skip rule, and a batch of long
+    reads only (no short-read bin beside the launch)."""
+    announce("long-read launch edge cases")
+    ref_size = 3_000_000
+    ref = M.synth_reference(46, ref_size)
+    parts = [M.synth_batch(M.make_spec(47 + k, n, length=length, sub=0.05, ins=0.02, dele=0.02,
+                                       candidates=c, distance=0.1), ref, ref_size)
+             for k, (n, length, c) in enumerate([(50, 300, 4), (3, 6000, 2), (3, 12000, 2), (2, 19000, 2)])]
+    b = concat_batches(parts)
+    with M.Myers(ref, ref_size, avg_query=800, cands_per_query=3, buffer_bytes=256 << 20) as m:
+        # (1) the 12 kb reads keep their Peq and qinfo but lose their candidates
+        keep = b["qinfo"][b["cands"]["query"], 1] != 12000
+        b1 = dict(b, cands=b["cands"][keep])
+        got = m.verify(b1["peq"], b1["qinfo"], b1["cands"], 0.1, flags=M.ENGINE_PERSISTENT)
+        np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b1, 0.1, O.MODE_ALIGNED))
+        # (2) every long candidate past the end of the reference
+        c2 = b["cands"].copy()
+        long_c = b["qinfo"][c2["query"], 1] > 1000
+        c2["position"][long_c] = ref_size - 10
+        b2 = dict(b, cands=c2)
+        got = m.verify(b2["peq"], b2["qinfo"], b2["cands"], 0.1, flags=M.ENGINE_PERSISTENT)
+        np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b2, 0.1, O.MODE_ALIGNED))
+        assert np.all(got[long_c] == [0, 0xFFFFFFFF])
+        # (3) long reads only
+        b3 = concat_batches(parts[1:])
+        got = m.verify(b3["peq"], b3["qinfo"], b3["cands"], 0.1, flags=M.ENGINE_PERSISTENT)
+        assert m.stats().launches == 1
+        np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b3, 0.1, O.MODE_ALIGNED))

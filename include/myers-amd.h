@@ -222,6 +222,8 @@ typedef struct {
     uint64_t candidates;        /* verified */
     uint64_t maxBatchRequests;
     double   busyMs;            /* sum of batch kernel time */
+    double   admitMs;           /* server-thread time forming and sending batches */
+    double   completeMs;        /* server-thread time handing results to requests */
 } myersServeStats_t;
 
 void myersServeDefaultConfig(myersServeConfig_t *cfg);

@@ -123,6 +123,7 @@ static int complete_ready(server_t *sv)
         if (!s->busy) continue;
         int ready = myers_buffer_ready(s->buf);
         if (ready == 0) continue;
+        const uint64_t t0 = now_ns();
         int rc = myersReceiveEx(s->buf);
         const resEntry_t *res = getResultsBuffer(s->buf);
         const uint64_t t = now_ns();
@@ -142,6 +143,7 @@ static int complete_ready(server_t *sv)
         s->busy = 0;
         s->nreq = 0;
         progressed = 1;
+        sv->stats.completeMs += (double)(now_ns() - t0) / 1e6;
     }
     if (progressed) pthread_cond_broadcast(&sv->cv_done);
     return progressed;
@@ -150,6 +152,7 @@ static int complete_ready(server_t *sv)
 /* Form and launch one batch in a free slot; caller holds the lock. */
 static int launch_batch(server_t *sv, slot_t *s)
 {
+    const uint64_t t_start = now_ns();
     qryEntry_t *peq = getPEQBuffer(s->buf);
     qryInfo_t *qinfo = getPEQInfoBuffer(s->buf);
     candInfo_t *cands = getCandidatesBuffer(s->buf);
@@ -228,6 +231,7 @@ static int launch_batch(server_t *sv, slot_t *s)
     }
     s->busy = 1;
     sv->stats.batches++;
+    sv->stats.admitMs += (double)(now_ns() - t_start) / 1e6;
     if (s->nreq > sv->stats.maxBatchRequests) sv->stats.maxBatchRequests = s->nreq;
     return 1;
 }
@@ -372,9 +376,12 @@ int myersServeGetStats(void *server, myersServeStats_t *out)
 {
     server_t *sv = (server_t *)server;
     if (sv == NULL || out == NULL) return MYERS_E_INVALID_ARGUMENT;
+    /* callers built against an older, shorter struct get its fields only */
+    const size_t n = out->structSize && out->structSize < sizeof(*out) ? out->structSize : sizeof(*out);
     pthread_mutex_lock(&sv->mu);
-    *out = sv->stats;
+    memcpy(out, &sv->stats, n);
     pthread_mutex_unlock(&sv->mu);
+    out->structSize = (uint32_t)n;
     return MYERS_SUCCESS;
 }
 

@@ -80,7 +80,8 @@ class Request(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     _fields_ = [("structSize", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("requests", ctypes.c_uint64),
                 ("batches", ctypes.c_uint64), ("candidates", ctypes.c_uint64),
-                ("maxBatchRequests", ctypes.c_uint64), ("busyMs", ctypes.c_double)]
+                ("maxBatchRequests", ctypes.c_uint64), ("busyMs", ctypes.c_double),
+                ("admitMs", ctypes.c_double), ("completeMs", ctypes.c_double)]
 
 
 class SynthSpec(ctypes.Structure):
@@ -606,6 +607,7 @@ class Server:
 
     def stats(self):
         s = ServeStats()
+        s.structSize = ctypes.sizeof(s)
         check(lib().myersServeGetStats(self._h, ctypes.byref(s)), "myersServeGetStats")
         return s
 

@@ -296,11 +296,39 @@ def serving_pool(ref, ref_size, seed, n_requests):
     return reqs
 
 
+def concat_requests(reqs):
+    """One batch holding every request (Peq offsets and query ids rebased)."""
+    peqs, qis, cands, e, q = [], [], [], 0, 0
+    for peq, qi, c in reqs:
+        qi = qi.copy()
+        qi[:, 0] += np.uint32(e)
+        c = c.copy()
+        c["query"] += np.uint32(q)
+        peqs.append(peq)
+        qis.append(qi)
+        cands.append(c)
+        e += len(peq)
+        q += len(qi)
+    return np.concatenate(peqs), np.concatenate(qis), np.concatenate(cands)
+
+
 def measure_serving(m, ref, ref_size, loads, seconds, batch_cells=0, engine=0, seed=5):
     pool = serving_pool(ref, ref_size, seed, 192)
     cells = [int((r[1][r[2]["query"], 1].astype(np.int64) * window_len(r[1][r[2]["query"], 1])).sum()) for r in pool]
     cands = [len(r[2]) for r in pool]
     alone = [m.verify(*r, DISTANCE) for r in pool[:24]]          # reference results for the mix check
+    # bulk rate of the same mix: the whole pool as one batch (host copies included, as in serving)
+    whole = concat_requests(pool)
+    bulk = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        whole_res = m.verify(*whole, DISTANCE)
+        bulk.append(len(whole[2]) / (time.perf_counter() - t0))
+    bulk_cps = max(bulk)
+    offs = np.concatenate([[0], np.cumsum(cands)])
+    for k in range(24):
+        if not np.array_equal(whole_res[offs[k]:offs[k + 1]], alone[k]):
+            raise SystemExit("bench: results of the pool as one batch differ from the requests verified alone")
     out = {"pool_requests": len(pool), "mean_reads_per_request": float(np.mean([len(r[1]) for r in pool])),
            "mean_candidates_per_request": float(np.mean(cands)), "buffers": m.num_buffers,
            "max_batch_cells": batch_cells or "library default (2.5e11)", "engine": engine, "levels": []}
@@ -310,7 +338,9 @@ def measure_serving(m, ref, ref_size, loads, seconds, batch_cells=0, engine=0, s
         lat, wall = sv.run_open_loop([x[0] for x in made], np.zeros(len(made), dtype=np.uint64))
         sat_cps = sum(cands) / (wall / 1e3)
         sat_rps = len(pool) / (wall / 1e3)
+        out["bulk_candidates_per_s"] = bulk_cps
         out["saturated_candidates_per_s"] = sat_cps
+        out["saturated_of_bulk"] = sat_cps / bulk_cps
         out["saturated_gcups"] = sum(cells) / (wall / 1e3) / 1e9
         for (req, keep), a in zip(made[:24], alone):
             if not np.array_equal(keep[3], a):
@@ -322,20 +352,28 @@ def measure_serving(m, ref, ref_size, loads, seconds, batch_cells=0, engine=0, s
             idx = rng.integers(0, len(pool), size=n)
             arrivals = (np.cumsum(rng.exponential(1.0 / rate, size=n)) * 1e9).astype(np.uint64)
             made = [M.Server.request(*pool[i]) for i in idx]
-            lat, wall = sv.run_open_loop([x[0] for x in made], arrivals)
+            lat, wall, qms = sv.run_open_loop([x[0] for x in made], arrivals, queue=True)
+            tail = lat >= np.percentile(lat, 99)
             for (req, keep), i in zip(made, idx):
                 if i < len(alone) and not np.array_equal(keep[3], alone[i]):
                     raise SystemExit("bench: served results differ from the same requests verified alone")
             nc = sum(cands[i] for i in idx)
             out["levels"].append({
                 "load": load, "requests": n, "offered_requests_per_s": rate,
+                "offered_of_bulk": rate * float(np.mean(cands)) / bulk_cps,
                 "candidates_per_s": nc / (wall / 1e3), "gcups": sum(cells[i] for i in idx) / (wall / 1e3) / 1e9,
                 "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                 "p999_ms": float(np.percentile(lat, 99.9)), "max_ms": float(lat.max()),
+                "queue_p50_ms": float(np.percentile(qms, 50)), "queue_p99_ms": float(np.percentile(qms, 99)),
+                "tail_queue_ms": float(qms[tail].mean()),
+                "tail_reads": float(np.mean([len(pool[i][1]) for i in idx[tail]])),
                 "max_reads_per_request": int(max(len(pool[i][1]) for i in idx)),
                 "batches": int(sv.stats().batches)})
-            log(f"serving load {load:.2f}: {out['levels'][-1]['candidates_per_s']:.3e} cand/s, "
-                f"p50 {out['levels'][-1]['p50_ms']:.2f} ms, p99 {out['levels'][-1]['p99_ms']:.2f} ms")
+            log(f"serving load {load:.2f} ({out['levels'][-1]['offered_of_bulk']:.2f} of bulk): "
+                f"{out['levels'][-1]['candidates_per_s']:.3e} cand/s, "
+                f"p50 {out['levels'][-1]['p50_ms']:.2f} ms, p99 {out['levels'][-1]['p99_ms']:.2f} ms, "
+                f"queue p99 {out['levels'][-1]['queue_p99_ms']:.2f} ms, slowest 1%: queue "
+                f"{out['levels'][-1]['tail_queue_ms']:.2f} ms, {out['levels'][-1]['tail_reads']:.0f} reads")
     return out
 
 

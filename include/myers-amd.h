@@ -243,6 +243,11 @@ int  myersServeStop(void *server);
  * wait for all; latencyMs[i] = completion - arrival. */
 int  myersServeRunOpenLoop(void *server, uint32_t numRequests, const myersRequest_t *reqs,
                            const uint64_t *arrivalNs, double *latencyMs, double *wallMs);
+/* As myersServeRunOpenLoop, and when queueMs is not NULL, queueMs[i] = time
+ * from arrival to the start of forming the batch that took request i (the
+ * rest of latencyMs[i] is that batch's copy, launch and verification). */
+int  myersServeRunOpenLoopEx(void *server, uint32_t numRequests, const myersRequest_t *reqs,
+                             const uint64_t *arrivalNs, double *latencyMs, double *queueMs, double *wallMs);
 
 #ifdef __cplusplus
 }

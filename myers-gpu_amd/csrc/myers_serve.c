@@ -36,6 +36,7 @@ typedef struct req {
     myersRequest_t r;
     uint64_t       ticket;
     uint64_t       t_submit_ns;
+    uint64_t       t_admit_ns;
     uint64_t       t_done_ns;
     uint64_t       bytes;
     uint64_t       cells;
@@ -171,6 +172,7 @@ static int launch_batch(server_t *sv, slot_t *s)
             if (!sv->head) sv->tail = NULL;
             q->error = MYERS_E_CAPACITY;
             q->t_done_ns = now_ns();
+            q->t_admit_ns = q->t_done_ns;
             q->state = REQ_DONE;
             pthread_cond_broadcast(&sv->cv_done);
             continue;
@@ -213,6 +215,7 @@ static int launch_batch(server_t *sv, slot_t *s)
         cells += q->cells;
         s->bytes += q->bytes;
         sv->inflight_bytes += q->bytes;
+        q->t_admit_ns = t_start;
         q->state = REQ_ADMITTED;
     }
     if (s->nreq == 0) return 0;
@@ -350,12 +353,13 @@ int myersServeSubmit(void *server, const myersRequest_t *req, uint64_t *ticket)
     return MYERS_SUCCESS;
 }
 
-static int wait_locked(server_t *sv, uint64_t ticket, uint64_t *t_submit, uint64_t *t_done)
+static int wait_locked(server_t *sv, uint64_t ticket, uint64_t *t_submit, uint64_t *t_admit, uint64_t *t_done)
 {
     if (ticket >= sv->next_ticket) return MYERS_E_INVALID_ARGUMENT;
     req_t *q = sv->table[ticket];
     while (q->state != REQ_DONE) pthread_cond_wait(&sv->cv_done, &sv->mu);
     if (t_submit) *t_submit = q->t_submit_ns;
+    if (t_admit) *t_admit = q->t_admit_ns;
     if (t_done) *t_done = q->t_done_ns;
     return q->error;
 }
@@ -366,7 +370,7 @@ int myersServeWait(void *server, uint64_t ticket, double *latencyMs)
     if (sv == NULL) return MYERS_E_INVALID_ARGUMENT;
     uint64_t ts = 0, td = 0;
     pthread_mutex_lock(&sv->mu);
-    int rc = wait_locked(sv, ticket, &ts, &td);
+    int rc = wait_locked(sv, ticket, &ts, NULL, &td);
     pthread_mutex_unlock(&sv->mu);
     if (latencyMs) *latencyMs = (double)(td - ts) / 1e6;
     return rc;
@@ -438,6 +442,12 @@ static void *submitter_main(void *arg)
 int myersServeRunOpenLoop(void *server, uint32_t numRequests, const myersRequest_t *reqs,
                           const uint64_t *arrivalNs, double *latencyMs, double *wallMs)
 {
+    return myersServeRunOpenLoopEx(server, numRequests, reqs, arrivalNs, latencyMs, NULL, wallMs);
+}
+
+int myersServeRunOpenLoopEx(void *server, uint32_t numRequests, const myersRequest_t *reqs,
+                            const uint64_t *arrivalNs, double *latencyMs, double *queueMs, double *wallMs)
+{
     server_t *sv = (server_t *)server;
     if (sv == NULL || (numRequests && (!reqs || !arrivalNs || !latencyMs))) return MYERS_E_INVALID_ARGUMENT;
     for (uint32_t i = 1; i < numRequests; i++)
@@ -464,12 +474,14 @@ int myersServeRunOpenLoop(void *server, uint32_t numRequests, const myersRequest
     for (uint32_t i = 0; i < numRequests; i++) {
         if (ol.tickets[i] == UINT64_MAX) {
             latencyMs[i] = -1.0;
+            if (queueMs) queueMs[i] = -1.0;
             continue;
         }
-        uint64_t td = 0;
-        int e = wait_locked(sv, ol.tickets[i], NULL, &td);
+        uint64_t ta = 0, td = 0;
+        int e = wait_locked(sv, ol.tickets[i], NULL, &ta, &td);
         if (e != MYERS_SUCCESS && rc == MYERS_SUCCESS) rc = e;
         latencyMs[i] = (double)(td - (ol.t0 + arrivalNs[i])) / 1e6;
+        if (queueMs) queueMs[i] = (double)(int64_t)(ta - (ol.t0 + arrivalNs[i])) / 1e6;
         if (td > last_done) last_done = td;
     }
     pthread_mutex_unlock(&sv->mu);

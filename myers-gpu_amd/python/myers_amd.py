@@ -180,6 +180,9 @@ def lib():
         L.myersServeStop.argtypes = [vp]
         L.myersServeRunOpenLoop.argtypes = [vp, u32, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64),
                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.myersServeRunOpenLoopEx.argtypes = [vp, u32, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64),
+                                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_double)]
         L.myersDefaultConfig.argtypes = [ctypes.POINTER(Config)]
         L.myersDefaultConfig.restype = None
         _lib = L
@@ -594,16 +597,21 @@ class Server:
         check(lib().myersServeWait(self._h, ticket, ctypes.byref(lat)), "myersServeWait")
         return lat.value
 
-    def run_open_loop(self, reqs, arrivals_ns):
+    def run_open_loop(self, reqs, arrivals_ns, queue=False):
+        """Latencies (ms) and wall time (ms); with queue=True also each
+        request's wait from arrival to admission into a batch (ms)."""
         n = len(reqs)
         arr = (Request * n)(*reqs)
         times = np.ascontiguousarray(arrivals_ns, dtype=np.uint64)
         lat = np.zeros(n, dtype=np.float64)
+        qms = np.zeros(n, dtype=np.float64)
         wall = ctypes.c_double()
-        check(lib().myersServeRunOpenLoop(self._h, n, arr, times.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
-                                          lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(wall)),
-              "myersServeRunOpenLoop")
-        return lat, wall.value
+        dp = ctypes.POINTER(ctypes.c_double)
+        check(lib().myersServeRunOpenLoopEx(self._h, n, arr, times.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                            lat.ctypes.data_as(dp), qms.ctypes.data_as(dp) if queue else None,
+                                            ctypes.byref(wall)),
+              "myersServeRunOpenLoopEx")
+        return (lat, wall.value, qms) if queue else (lat, wall.value)
 
     def stats(self):
         s = ServeStats()

@@ -81,8 +81,9 @@ def test_open_loop_poisson():
             made = [M.Server.request(*r) for r in reqs * 3]
             gaps = rng.exponential(1e6 / 2000.0, size=len(made))          # 2000 requests/s
             arrivals = (np.cumsum(gaps) * 1e3).astype(np.uint64)
-            lat, wall = sv.run_open_loop([x[0] for x in made], arrivals)
+            lat, wall, queue = sv.run_open_loop([x[0] for x in made], arrivals, queue=True)
         assert np.all(lat >= 0) and wall > 0
+        assert np.all(queue >= 0) and np.all(queue <= lat)      # admitted before completion
         for (req, keep), r in zip(made, reqs * 3):
             np.testing.assert_array_equal(keep[3], oracle_req(ref, r))
 

@@ -79,7 +79,7 @@ int myers_variant_occupancy_persistent(uint32_t v, int *wg_per_cu);
  * longest items on other CUs; later items are pulled from one counter each
  * (queue_heads + MYERS_WIDE_HEAD / MYERS_LONG_HEAD, reset by the binning).
  * Needs bin_ranges and queue_heads. */
-enum { MYERS_LONG_MAX = 16, MYERS_LONG_WAVES = 4 };
+enum { MYERS_LONG_MAX = 24, MYERS_LONG_WAVES = 4 };
 #define MYERS_LONG_HEAD ((MYERS_MAX_VARIANTS - 1) * MYERS_QUEUE_PARTS * MYERS_QUEUE_STRIDE)
 #define MYERS_WIDE_HEAD (MYERS_LONG_HEAD + MYERS_QUEUE_STRIDE)
 /* LDS of one workgroup-wide candidate of the long-read launch (bytes), and

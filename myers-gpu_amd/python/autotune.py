@@ -24,7 +24,10 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import myers_amd as M  # noqa: E402
 
 SHORT = [64, 100, 128, 160, 200, 256, 320, 400, 480, 600, 700, 800, 900, 1000]
-LONG = [1500, 2000, 3000, 4000, 5000, 6500, 8000, 10000, 13000, 16000, 20000, 24576]
+# Long reads: one length at each capacity (32 * lanes * words bases) of the
+# multi-lane variants, so that the variant chosen at a capacity serves exactly
+# the reads between the previous capacity and it.
+LONG = [1536, 2048, 3072, 4096, 5120, 6144, 8192, 10240, 12288, 16384, 20480, 24576]
 CHUNK_FACTORS = (1, 2, 4)
 REF_SIZE = 200_000_000
 # Each measurement must run in the throughput regime: enough candidates to

@@ -60,7 +60,7 @@ DISTANCE = 0.1
 # HBM bytes per launch of the default workload (config 2, aligned windows),
 # from FETCH_SIZE and WRITE_SIZE counter runs scaled by a 1 GiB copy
 # calibration (profiles/hbm_traffic_verify_L1W32_config2.md).
-MEASURED_TRAFFIC = {(2, "aligned"): 1224.7e6}
+MEASURED_TRAFFIC = {(2, "aligned"): 1423.9e6}
 
 
 def log(msg):

@@ -169,7 +169,13 @@ __device__ __forceinline__ uint32_t from_lower_lane(uint32_t v, uint32_t not_fir
 // test is made on a VGPR copy of ga (ga_v), which compiles to an exec mask
 // and a skip branch that keep every word in its register.  ph_top / mh_top
 // are then those of group ga-1's top word.
-template <int W, bool kHin, bool kCut, uint32_t kGS>
+//
+// kPhased: issue each group's word updates in phases (below), which pairs
+// double-rate operations across waves for throughput; a wave that runs a long
+// chain alone on its SIMD (the long-read launch, multi-wave candidates) is
+// faster with the compiler's interleaved order, which overlaps the carry
+// chain with the other words' operations.
+template <int W, bool kHin, bool kCut, uint32_t kGS, bool kPhased = true>
 __device__ __forceinline__ void column_grp(uint32_t (&Pv)[W], uint32_t (&Mv)[W], const uint4 &e0,
                                            const uint4 *lds_lane, uint32_t cur_off, uint32_t ga_v,
                                            uint32_t ph_in, uint32_t mh_in, uint32_t &ph_top, uint32_t &mh_top) {
@@ -181,56 +187,77 @@ __device__ __forceinline__ void column_grp(uint32_t (&Pv)[W], uint32_t (&Mv)[W],
         if (!kCut || g == 0 || (uint32_t)g < ga_v) {
             if (g + 1 < kG) nxt = lds_lane[cur_off + (uint32_t)(g + 1) * kGS];
             const uint32_t e4[4] = {cur.x, cur.y, cur.z, cur.w};
-            const int k0 = 4 * g;
-            const int n = (W - k0) < 4 ? (W - k0) : 4;
-            uint32_t eqv[4], t[4], xv[4], sum[4], xh[4], ph[4], mh[4], php[4], mhp[4];
-            // The group's words go through the recurrence in phases, each
-            // phase over all of its words, so that the double-rate bit
-            // operations of independent words issue back to back (a SIMD
-            // issues two of them in one quad-cycle) and the single-rate ones
-            // (add with carry, funnel shifts) come in runs:
-            //   Eq & Pv, Xv | carry chain | Xh, Ph, Mh | shifts | Pv, Mv.
-            // sched_barrier keeps the compiler from interleaving the phases.
+            if constexpr (kPhased) {
+                const int k0 = 4 * g;
+                const int n = (W - k0) < 4 ? (W - k0) : 4;
+                uint32_t eqv[4], t[4], xv[4], sum[4], xh[4], ph[4], mh[4], php[4], mhp[4];
+                // The group's words go through the recurrence in phases, each
+                // phase over all of its words, so that the double-rate bit
+                // operations of independent words issue back to back (a SIMD
+                // issues two of them in one quad-cycle) and the single-rate ones
+                // (add with carry, funnel shifts) come in runs:
+                //   Eq & Pv, Xv | carry chain | Xh, Ph, Mh | shifts | Pv, Mv.
+                // sched_barrier keeps the compiler from interleaving the phases.
+    #pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (j >= n) break;
+                    const int k = k0 + j;
+                    eqv[j] = (kHin && k == 0) ? (e4[0] | (mh_in >> 31)) : e4[j];
+                    t[j] = eqv[j] & Pv[k];
+                    xv[j] = e4[j] | Mv[k];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+    #pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (j >= n) break;
+                    uint32_t c_out;
+                    sum[j] = __builtin_addc(t[j], Pv[k0 + j], carry, &c_out);
+                    carry = c_out;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+    #pragma unroll
+                for (int j = 0; j < 4 && j < n; j++) xh[j] = __builtin_amdgcn_bitop3_b32(sum[j], Pv[k0 + j], eqv[j], kXorOr);
+    #pragma unroll
+                for (int j = 0; j < 4 && j < n; j++) {
+                    ph[j] = __builtin_amdgcn_bitop3_b32(Mv[k0 + j], xh[j], Pv[k0 + j], kOrNor);
+                    mh[j] = Pv[k0 + j] & xh[j];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+    #pragma unroll
+                for (int j = 0; j < 4 && j < n; j++) {
+                    const int k = k0 + j;
+                    php[j] = (!kHin && k == 0) ? (ph[j] << 1) : __builtin_amdgcn_alignbit(ph[j], ph_prev, 31u);
+                    mhp[j] = (!kHin && k == 0) ? (mh[j] << 1) : __builtin_amdgcn_alignbit(mh[j], mh_prev, 31u);
+                    ph_prev = ph[j];
+                    mh_prev = mh[j];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+    #pragma unroll
+                for (int j = 0; j < 4 && j < n; j++) {
+                    Pv[k0 + j] = __builtin_amdgcn_bitop3_b32(mhp[j], xv[j], php[j], kOrNor);
+                    Mv[k0 + j] = php[j] & xv[j];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            } else {
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                if (j >= n) break;
-                const int k = k0 + j;
-                eqv[j] = (kHin && k == 0) ? (e4[0] | (mh_in >> 31)) : e4[j];
-                t[j] = eqv[j] & Pv[k];
-                xv[j] = e4[j] | Mv[k];
+                for (int k = 4 * g; k < 4 * g + 4 && k < W; k++) {
+                    const uint32_t pv = Pv[k], mv = Mv[k];
+                    const uint32_t xv = e4[k - 4 * g] | mv;
+                    const uint32_t eq = (kHin && k == 0) ? (e4[0] | (mh_in >> 31)) : e4[k - 4 * g];
+                    uint32_t c_out;
+                    const uint32_t sum = __builtin_addc(eq & pv, pv, carry, &c_out);
+                    carry = c_out;
+                    const uint32_t xh = __builtin_amdgcn_bitop3_b32(sum, pv, eq, kXorOr);
+                    const uint32_t ph = __builtin_amdgcn_bitop3_b32(mv, xh, pv, kOrNor);
+                    const uint32_t mh = pv & xh;
+                    const uint32_t php = (!kHin && k == 0) ? (ph << 1) : __builtin_amdgcn_alignbit(ph, ph_prev, 31u);
+                    const uint32_t mhp = (!kHin && k == 0) ? (mh << 1) : __builtin_amdgcn_alignbit(mh, mh_prev, 31u);
+                    ph_prev = ph;
+                    mh_prev = mh;
+                    Pv[k] = __builtin_amdgcn_bitop3_b32(mhp, xv, php, kOrNor);
+                    Mv[k] = php & xv;
+                }
             }
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                if (j >= n) break;
-                uint32_t c_out;
-                sum[j] = __builtin_addc(t[j], Pv[k0 + j], carry, &c_out);
-                carry = c_out;
-            }
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int j = 0; j < 4 && j < n; j++) xh[j] = __builtin_amdgcn_bitop3_b32(sum[j], Pv[k0 + j], eqv[j], kXorOr);
-#pragma unroll
-            for (int j = 0; j < 4 && j < n; j++) {
-                ph[j] = __builtin_amdgcn_bitop3_b32(Mv[k0 + j], xh[j], Pv[k0 + j], kOrNor);
-                mh[j] = Pv[k0 + j] & xh[j];
-            }
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int j = 0; j < 4 && j < n; j++) {
-                const int k = k0 + j;
-                php[j] = (!kHin && k == 0) ? (ph[j] << 1) : __builtin_amdgcn_alignbit(ph[j], ph_prev, 31u);
-                mhp[j] = (!kHin && k == 0) ? (mh[j] << 1) : __builtin_amdgcn_alignbit(mh[j], mh_prev, 31u);
-                ph_prev = ph[j];
-                mh_prev = mh[j];
-            }
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int j = 0; j < 4 && j < n; j++) {
-                Pv[k0 + j] = __builtin_amdgcn_bitop3_b32(mhp[j], xv[j], php[j], kOrNor);
-                Mv[k0 + j] = php[j] & xv[j];
-            }
-            __builtin_amdgcn_sched_barrier(0);
             cur = nxt;
         }
     }
@@ -497,7 +524,7 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
                         const uint4 eq4n = lds_lane[next_off];        // next column, group 0
                         const uint32_t ph_in = from_lower_lane<L>(ph_top, not_first);
                         const uint32_t mh_in = from_lower_lane<L>(mh_top, not_first);
-                        column_grp<W, true, false, (uint32_t)S::kGroupU4>(
+                        column_grp<W, true, false, (uint32_t)S::kGroupU4, kWaves == 1>(
                             Pv, Mv, eq4, lds_lane, cur_off, 0u, ph_in, mh_in, ph_top, mh_top);
                         eq4 = eq4n;
                         // score row and first minimum (src-api/coreMyersGPU-CC35.cu:205-207,227-228);
@@ -792,7 +819,7 @@ __device__ __forceinline__ uint2 wide_columns(WideWorkspace<Q, W> &ws, uint32_t 
             }
             const uint32_t ph_in = (uint32_t)__builtin_amdgcn_update_dpp((int)fill_p, (int)ph_top, 0x138, 0xF, 0xF, false);
             const uint32_t mh_in = (uint32_t)__builtin_amdgcn_update_dpp((int)fill_m, (int)mh_top, 0x138, 0xF, 0xF, false);
-            column_grp<W, true, false, kGS>(Pv, Mv, eq4, lds_lane, cur_off, 0u, ph_in, mh_in, ph_top, mh_top);
+            column_grp<W, true, false, kGS, false>(Pv, Mv, eq4, lds_lane, cur_off, 0u, ph_in, mh_in, ph_top, mh_top);
             eq4 = eq4n;
             if constexpr (kAbove) {
                 acc_p = (acc_p >> 1) | (ph_top & 0x80000000u);

@@ -27,7 +27,7 @@ SHORT = [64, 100, 128, 160, 200, 256, 320, 400, 480, 600, 700, 800, 900, 1000]
 # Long reads: one length at each capacity (32 * lanes * words bases) of the
 # multi-lane variants, so that the variant chosen at a capacity serves exactly
 # the reads between the previous capacity and it.
-LONG = [1536, 2048, 3072, 4096, 5120, 6144, 8192, 10240, 12288, 16384, 20480, 24576]
+LONG = [1536, 2048, 3072, 4096, 5120, 6144, 7168, 8192, 10240, 12288, 14336, 16384, 20480, 24576]
 CHUNK_FACTORS = (1, 2, 4)
 REF_SIZE = 200_000_000
 # Each measurement must run in the throughput regime: enough candidates to

@@ -170,11 +170,12 @@ __device__ __forceinline__ uint32_t from_lower_lane(uint32_t v, uint32_t not_fir
 // and a skip branch that keep every word in its register.  ph_top / mh_top
 // are then those of group ga-1's top word.
 //
-// kPhased: issue each group's word updates in phases (below), which pairs
-// double-rate operations across waves for throughput; a wave that runs a long
-// chain alone on its SIMD (the long-read launch, multi-wave candidates) is
-// faster with the compiler's interleaved order, which overlaps the carry
-// chain with the other words' operations.
+// kPhased: issue each group's word updates in phases (below).  With several
+// waves per SIMD this raises the share of quad-cycles that issue two VALU
+// instructions (18% -> 35% on 1000-base reads, 11% faster); a wave that runs
+// a long chain alone on its SIMD (the long-read launch, multi-wave
+// candidates) is 5-10% faster with the compiler's interleaved order, which
+// overlaps the carry chain with the other words' operations.
 template <int W, bool kHin, bool kCut, uint32_t kGS, bool kPhased = true>
 __device__ __forceinline__ void column_grp(uint32_t (&Pv)[W], uint32_t (&Mv)[W], const uint4 &e0,
                                            const uint4 *lds_lane, uint32_t cur_off, uint32_t ga_v,

@@ -34,7 +34,9 @@ typedef enum {
     MYERS_E_QUERY_TOO_LONG       = 19,  /* a query exceeds the longest supported length;
                                            its candidates got the skip sentinel */
     MYERS_E_NO_DEVICE            = 20,  /* no HIP device visible */
-    MYERS_E_REFERENCE_TOO_LARGE  = 21   /* reference words do not fit 32-bit indexing */
+    MYERS_E_REFERENCE_TOO_LARGE  = 21   /* stored words + 64 guard words exceed 2^31 - 1
+                                           (about 17 Gbases): the kernels index words
+                                           with signed 32-bit offsets */
 } myersError_t;
 
 /* Window semantics (flags of myersSendEx / myersVerifyDevice). */

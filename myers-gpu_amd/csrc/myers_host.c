@@ -234,10 +234,21 @@ static int read_reference_file(const char *fn, uint32_t **words, uint64_t *num_w
     return MYERS_SUCCESS;
 }
 
+/* The kernels index reference words with signed 32-bit offsets (a lane's
+ * first text word can lie before the window, and reads are clamped to
+ * [0, last word]), so the stored words plus the guard must stay below 2^31:
+ * 17 Gbases, four times the largest .4bits.ref header (u32 base count). */
+static int check_reference_size(uint64_t num_words, uint64_t size)
+{
+    if (num_words + GUARD_WORDS > (uint64_t)INT32_MAX) return MYERS_E_REFERENCE_TOO_LARGE;
+    if (size > num_words * 8u) return MYERS_E_INVALID_ARGUMENT;
+    return MYERS_SUCCESS;
+}
+
 static int upload_reference(myers_ctx_t *ctx, const uint32_t *words, uint64_t num_words, uint64_t size)
 {
-    if (num_words + GUARD_WORDS >= 0xFFFFFFF0ull) return MYERS_E_REFERENCE_TOO_LARGE;
-    if (size > num_words * 8u) return MYERS_E_INVALID_ARGUMENT;
+    int rc0 = check_reference_size(num_words, size);
+    if (rc0) return rc0;
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
     const uint64_t bytes = (num_words + GUARD_WORDS) * sizeof(uint32_t);
@@ -416,12 +427,14 @@ static int init_common(void ***out, uint32_t num_buffers, const uint32_t *words,
                        uint64_t size, int32_t avg_query, int32_t cands_per_query, const myersConfig_t *cfg_in)
 {
     if (out == NULL || num_buffers == 0 || words == NULL) return MYERS_E_INVALID_ARGUMENT;
+    int rc = check_reference_size(num_words, size);      /* before any device work */
+    if (rc) return rc;
     myersConfig_t cfg;
     myersDefaultConfig(&cfg);
     if (cfg_in) cfg = *cfg_in;
     *out = NULL;
     myers_ctx_t *ctx = NULL;
-    int rc = create_context(&ctx, cfg.device);
+    rc = create_context(&ctx, cfg.device);
     if (rc) return rc;
     rc = upload_reference(ctx, words, num_words, size);
     if (rc) { free_context(ctx); return rc; }

@@ -17,8 +17,10 @@ import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_DIR = os.path.join(PKG_DIR, "lib")
-LIB_PATH = os.path.join(LIB_DIR, "libmyers_amd.so")
-TOOLS_PATH = os.path.join(LIB_DIR, "libmyers_tools.so")
+# MYERS_AMD_LIB / MYERS_TOOLS_LIB select other builds of the two libraries
+# (the CPU sanitizer builds of `make -C myers-gpu_amd san`).
+LIB_PATH = os.environ.get("MYERS_AMD_LIB") or os.path.join(LIB_DIR, "libmyers_amd.so")
+TOOLS_PATH = os.environ.get("MYERS_TOOLS_LIB") or os.path.join(LIB_DIR, "libmyers_tools.so")
 
 CAND_DTYPE = np.dtype([("query", "<u4"), ("pad", "<u4"), ("position", "<u8")])
 SKIP_SCORE = 0xFFFFFFFF
@@ -35,6 +37,7 @@ ERRORS = {
     4: "E_ALLOCATE_MEM", 5: "E_INCOMPATIBLE_GPU", 16: "E_INVALID_ARGUMENT", 17: "E_CAPACITY",
     18: "E_HIP", 19: "E_QUERY_TOO_LONG", 20: "E_NO_DEVICE", 21: "E_REFERENCE_TOO_LARGE",
 }
+globals().update({name: code for code, name in ERRORS.items()})
 
 
 class MyersError(RuntimeError):
@@ -412,6 +415,7 @@ class Myers:
                                        ctypes.byref(cfg))
         check(rc, "myersInit")
         self.num_buffers = num_buffers
+        self._last = [0] * num_buffers       # candidates of each buffer's last send
 
     def buffer(self, i=0):
         return self._handles[i]
@@ -442,11 +446,11 @@ class Myers:
         v["cands"][:len(cands)] = cands
         check(lib().myersSendEx(self.buffer(i), distance, len(peq), len(qinfo), len(cands), flags),
               "myersSendEx")
-        self._last = len(cands)
+        self._last[i] = len(cands)
 
     def receive(self, i=0, count=None):
         rc = lib().myersReceiveEx(self.buffer(i))
-        res = self.views(i)["results"][: (count if count is not None else self._last)].copy()
+        res = self.views(i)["results"][: (count if count is not None else self._last[i])].copy()
         check(rc, "myersReceiveEx")
         return res
 

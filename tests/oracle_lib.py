@@ -11,7 +11,7 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(REPO, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+ORACLE_SO = os.environ.get("MYERS_ORACLE_SO") or os.path.join(ORACLE_DIR, "liboracle.so")
 
 MODE_ALIGNED = 0
 MODE_EXACT = 1

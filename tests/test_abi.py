@@ -144,3 +144,22 @@ def test_c_caller_end_to_end():
             got[i] = (c, s)
     exp = O.myers_batch(ref, ref_size, b["peq"], b["qinfo"], b["cands"], 0.1, 0)
     np.testing.assert_array_equal(got, exp)
+
+
+def test_reference_size_limit_before_device_work():
+    """The kernels index reference words with signed 32-bit offsets, so
+    myersInitFromMemory refuses 2^31 - 64 words or more (with the 64 guard
+    words) before it touches the device; the same call runs here, without a
+    GPU, and the refused sizes never read the word array."""
+    words = np.zeros(4, dtype=np.uint32)
+    handles = ctypes.POINTER(ctypes.c_void_p)()
+    L = M.lib()
+    for nw in (2**31 - 64, 2**31, 2**32 - 1):
+        rc = L.myersInitFromMemory(ctypes.byref(handles), 1, words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                   nw, 8 * nw, 1000, 11, None)
+        assert rc == M.E_REFERENCE_TOO_LARGE, (nw, rc)
+        assert not handles
+    # a size larger than the words hold is an argument error, also before device work
+    rc = L.myersInitFromMemory(ctypes.byref(handles), 1, words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                               4, 33, 1000, 11, None)
+    assert rc == M.E_INVALID_ARGUMENT

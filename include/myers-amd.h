@@ -145,6 +145,19 @@ uint64_t    myersGetReferenceSize(void *myersBuffer);             /* extension *
 int myersVerifyDevice(void *myersBuffer, const void *d_peq, uint32_t numPEQEntries, const void *d_qinfo,
                       uint32_t numQueries, const void *d_candidates, uint32_t numCandidates,
                       float distance, uint32_t flags, void *d_results, const qryInfo_t *h_qinfo);
+/* myersVerifyDevice that allocates nothing (extension, no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * binning workspace is a static device array of the library, one per GPU and
+ * held until the process ends, instead of the buffer's workspace.  At most
+ * MYERS_NOALLOC_MAX_CANDIDATES candidates (more: MYERS_E_CAPACITY with a
+ * message); MYERS_USE_GRAPH is refused (MYERS_E_INVALID_ARGUMENT).  Every
+ * call on one GPU shares that array, so calls that overlap on one GPU --
+ * from different buffers or threads -- race: serialise them. */
+#define MYERS_NOALLOC_MAX_CANDIDATES (1u << 21)
+int myersVerifyDeviceNoAlloc(void *myersBuffer, const void *d_peq, uint32_t numPEQEntries, const void *d_qinfo,
+                             uint32_t numQueries, const void *d_candidates, uint32_t numCandidates,
+                             float distance, uint32_t flags, void *d_results, const qryInfo_t *h_qinfo);
 int myersSynchronize(void *myersBuffer);          /* wait for the buffer's stream */
 void *myersGetStream(void *myersBuffer);          /* the buffer's hipStream_t */
 

@@ -44,12 +44,6 @@
 #define DEFAULT_BUFFER_CAP (2ull << 30)
 #define EV_RING          32u            /* kernel-timing event pairs per buffer */
 #define SIDE_STREAMS     4u             /* fork width of a persistent launch */
-/* d_work: ranges (2x) | err | queue part heads | binning block counts */
-#define WORK_RANGES      0u
-#define WORK_ERR         (2u * MYERS_MAX_VARIANTS)
-#define WORK_HEADS       (8u * MYERS_MAX_VARIANTS)     /* 2 KB in: 128-byte aligned */
-#define WORK_BLOCKS      (WORK_HEADS + (uint32_t)MYERS_QUEUE_WORDS)
-#define WORK_WORDS       (WORK_BLOCKS + (uint32_t)MYERS_BIN_WORDS)
 
 typedef struct myers_ctx {
     int       device;
@@ -67,6 +61,7 @@ typedef struct myers_ctx {
     uint8_t   bin_of_words[MAX_QUERY_WORDS + 1];
     uint8_t  *d_bin_of_words;
     uint64_t  tune_gen;          /* bumped when the bin table or chunk factors change */
+    uint32_t *d_scratch;         /* this device's static workspace (myersVerifyDeviceNoAlloc) */
 } myers_ctx_t;
 
 typedef struct myers_buffer {
@@ -86,7 +81,7 @@ typedef struct myers_buffer {
     resEntry_t  *h_results;
     resEntry_t  *d_results;
     uint32_t    *d_order;
-    uint32_t    *d_work;          /* ranges(2x) | err | queue part heads | binning block counts */
+    uint32_t    *d_work;          /* MYERS_WORK_WORDS: ranges | err | queue part heads | block counts */
     uint32_t    *h_err;           /* pinned copy of the binning error flag */
     hipEvent_t   ev_batch_start, ev_batch_end;
     /* HIP events around the verification launches of each batch, in a ring
@@ -328,7 +323,7 @@ static int init_buffer(myers_buffer_t *b)
     HIP_TRY(hipMalloc((void **)&b->d_cands, ca_b));
     HIP_TRY(hipMalloc((void **)&b->d_results, re_b));
     HIP_TRY(hipMalloc((void **)&b->d_order, (size_t)b->max_candidates * sizeof(uint32_t)));
-    HIP_TRY(hipMalloc((void **)&b->d_work, WORK_WORDS * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc((void **)&b->d_work, MYERS_WORK_WORDS * sizeof(uint32_t)));
     for (uint32_t i = 0; i < SIDE_STREAMS; i++) {
         HIP_TRY(hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking));
         HIP_TRY(hipEventCreateWithFlags(&b->ev_join[i], hipEventDisableTiming));
@@ -358,6 +353,11 @@ static int create_context(myers_ctx_t **out, int device)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { free(ctx); return MYERS_E_HIP; }
     ctx->num_cus = prop.multiProcessorCount;
+    /* the allocation-free entry point's static workspace on this device */
+    if (myers_scratch_init(device, &ctx->d_scratch) != 0) {
+        free(ctx);
+        return MYERS_E_HIP;
+    }
     ctx->num_variants = myers_num_variants();
     for (uint32_t v = 0; v < ctx->num_variants; v++) {
         myers_variant_desc(v, &ctx->variants[v]);
@@ -609,6 +609,8 @@ typedef struct {
     uint32_t    num_entries, num_queries, num_cands, flags;
     float       distance;
     float       threshold;
+    uint32_t   *d_work;      /* MYERS_WORK_WORDS of pipeline workspace */
+    uint32_t   *d_order;     /* slot -> candidate, num_cands words */
 } batch_t;
 
 static uint32_t resolve_engine(uint32_t flags, const bin_scan_t *scan)
@@ -765,16 +767,17 @@ static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan
         return MYERS_SUCCESS;
     }
     /* device-side binning */
-    uint32_t *ranges = b->d_work + WORK_RANGES;
-    uint32_t *heads = b->d_work + WORK_HEADS;
-    uint32_t *err = b->d_work + WORK_ERR;
+    uint32_t *ranges = bt->d_work + MYERS_WORK_RANGES;
+    uint32_t *heads = bt->d_work + MYERS_WORK_HEADS;
+    uint32_t *err = bt->d_work + MYERS_WORK_ERR;
     rc = myers_launch_binning((const uint32_t *)bt->d_qinfo, bt->num_queries, bt->d_cands, bt->num_cands,
                               ctx->d_bin_of_words, max_supported_words(ctx), ctx->num_variants,
-                              b->d_work + WORK_BLOCKS, ranges, b->d_order, err, bt->d_results, heads, b->stream);
+                              bt->d_work + MYERS_WORK_BLOCKS, ranges, bt->d_order, err, bt->d_results, heads,
+                              b->stream);
     if (rc) return hip_fail((hipError_t)rc, "binning launch", __FILE__, __LINE__);
     HIP_TRY(hipMemcpyAsync(b->h_err, err, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
     b->binned = 1;
-    a.order = b->d_order;
+    a.order = bt->d_order;
     a.bin_ranges = ranges;
     a.queue_heads = heads;
     uint32_t order[MYERS_MAX_VARIANTS];
@@ -964,7 +967,7 @@ int myersSendEx(void *myersBuffer, float distance, uint32_t numPEQEntries, uint3
     if (numCandidates)
         SEND_TRY(hipMemcpyAsync(b->d_cands, b->h_cands, (size_t)numCandidates * sizeof(candInfo_t), hipMemcpyHostToDevice, s));
     const batch_t bt = {b->d_peq, b->d_qinfo, b->d_cands, b->d_results, numPEQEntries, numQueries,
-                        numCandidates, flags, distance, b->threshold};
+                        numCandidates, flags, distance, b->threshold, b->d_work, b->d_order};
     rc = enqueue_verify(b, &bt, b->h_qinfo, b->h_cands);
     if (rc) return (b->last_error = rc);
     if (numCandidates)
@@ -1031,6 +1034,24 @@ void receiveMyersBuffer(void *myersBuffer)
 
 /* ------------------------------------------------------- device batches */
 
+static int verify_device(myers_buffer_t *b, const void *d_peq, uint32_t numPEQEntries, const void *d_qinfo,
+                         uint32_t numQueries, const void *d_candidates, uint32_t numCandidates,
+                         float distance, uint32_t flags, void *d_results, const qryInfo_t *h_qinfo,
+                         uint32_t *d_work, uint32_t *d_order)
+{
+    int rc = hip_fail(hipSetDevice(b->ctx->device), "hipSetDevice", __FILE__, __LINE__);
+    if (rc) return (b->last_error = rc);
+    *b->h_err = 0;
+    const batch_t bt = {d_peq, d_qinfo, d_candidates, d_results, numPEQEntries, numQueries, numCandidates,
+                        flags, distance, b->threshold, d_work, d_order};
+    rc = enqueue_verify(b, &bt, h_qinfo, NULL);
+    b->stats.cells = 0;
+    b->stats.candidates = numCandidates;
+    b->pending = (rc == MYERS_SUCCESS);
+    b->last_error = rc;
+    return rc;
+}
+
 int myersVerifyDevice(void *myersBuffer, const void *d_peq, uint32_t numPEQEntries, const void *d_qinfo,
                       uint32_t numQueries, const void *d_candidates, uint32_t numCandidates,
                       float distance, uint32_t flags, void *d_results, const qryInfo_t *h_qinfo)
@@ -1041,17 +1062,32 @@ int myersVerifyDevice(void *myersBuffer, const void *d_peq, uint32_t numPEQEntri
         (numCandidates && (!d_peq || !d_qinfo || !d_candidates || !d_results)))
         return (b->last_error = MYERS_E_INVALID_ARGUMENT);
     if (numCandidates > b->max_candidates) return (b->last_error = MYERS_E_CAPACITY);
-    int rc = hip_fail(hipSetDevice(b->ctx->device), "hipSetDevice", __FILE__, __LINE__);
-    if (rc) return (b->last_error = rc);
-    *b->h_err = 0;
-    const batch_t bt = {d_peq, d_qinfo, d_candidates, d_results, numPEQEntries, numQueries, numCandidates,
-                        flags, distance, b->threshold};
-    rc = enqueue_verify(b, &bt, h_qinfo, NULL);
-    b->stats.cells = 0;
-    b->stats.candidates = numCandidates;
-    b->pending = (rc == MYERS_SUCCESS);
-    b->last_error = rc;
-    return rc;
+    return verify_device(b, d_peq, numPEQEntries, d_qinfo, numQueries, d_candidates, numCandidates, distance,
+                         flags, d_results, h_qinfo, b->d_work, b->d_order);
+}
+
+/* myersVerifyDevice with the pipeline workspace in the device's static
+ * array instead of the buffer's: nothing is allocated.  Every call on one
+ * GPU shares that array (the binning kernels re-initialise the parts that
+ * must start at zero), so calls that overlap on one GPU race. */
+int myersVerifyDeviceNoAlloc(void *myersBuffer, const void *d_peq, uint32_t numPEQEntries, const void *d_qinfo,
+                             uint32_t numQueries, const void *d_candidates, uint32_t numCandidates,
+                             float distance, uint32_t flags, void *d_results, const qryInfo_t *h_qinfo)
+{
+    myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
+    if (b == NULL) return MYERS_E_INVALID_ARGUMENT;
+    /* graph instantiation may allocate: not on this path */
+    if ((flags & ~(1u | MYERS_ENGINE_MASK)) ||
+        (numCandidates && (!d_peq || !d_qinfo || !d_candidates || !d_results)))
+        return (b->last_error = MYERS_E_INVALID_ARGUMENT);
+    if (numCandidates > MYERS_NOALLOC_MAX_CANDIDATES) {
+        fprintf(stderr, "Myers - myersVerifyDeviceNoAlloc: %u candidates exceed the static workspace "
+                "maximum of %u\n", numCandidates, (unsigned)MYERS_NOALLOC_MAX_CANDIDATES);
+        return (b->last_error = MYERS_E_CAPACITY);
+    }
+    uint32_t *scratch = b->ctx->d_scratch;
+    return verify_device(b, d_peq, numPEQEntries, d_qinfo, numQueries, d_candidates, numCandidates, distance,
+                         flags, d_results, h_qinfo, scratch, scratch + MYERS_WORK_WORDS);
 }
 
 int myersSetThreshold(void *myersBuffer, float errorRate)

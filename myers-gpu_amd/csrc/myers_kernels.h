@@ -119,6 +119,28 @@ int myers_launch_long(const myers_verify_args_t *args, const myers_long_plan_t *
  * num_queries gets the sentinel and sets err to 2. */
 enum { MYERS_BIN_BLOCKS = 4096 };
 #define MYERS_BIN_WORDS (MYERS_BIN_BLOCKS * MYERS_MAX_VARIANTS)
+
+/* Workspace of one batch's device pipeline, in u32 words: variant ranges
+ * (2 per variant) and the binning error flag in the first
+ * 8 * MYERS_MAX_VARIANTS words, then the persistent queues' part heads
+ * (128-byte aligned), then the binning block counts.  The slot -> candidate
+ * order (one word per candidate) is separate. */
+#define MYERS_WORK_RANGES 0u
+#define MYERS_WORK_ERR    (2u * MYERS_MAX_VARIANTS)
+#define MYERS_WORK_HEADS  (8u * MYERS_MAX_VARIANTS)
+#define MYERS_WORK_BLOCKS (MYERS_WORK_HEADS + (uint32_t)MYERS_QUEUE_WORDS)
+#define MYERS_WORK_WORDS  (MYERS_WORK_BLOCKS + (uint32_t)MYERS_BIN_WORDS)
+
+/* Static device workspace of the allocation-free entry point
+ * (myersVerifyDeviceNoAlloc): a file-scope __device__ array in the kernel
+ * library holding MYERS_WORK_WORDS workspace words followed by the order of
+ * up to MYERS_NOALLOC_MAX_CANDIDATES candidates (9.1 MB per GPU, held until
+ * the process ends).  myers_scratch_init resolves its device address on
+ * `device`, which must be the current device, once per device (later calls
+ * return the kept address); 0 on success, else a hipError_t. */
+#define MYERS_NOALLOC_MAX_CANDIDATES (1u << 21)
+#define MYERS_SCRATCH_WORDS (MYERS_WORK_WORDS + MYERS_NOALLOC_MAX_CANDIDATES)
+int myers_scratch_init(int device, uint32_t **ptr);
 int myers_launch_binning(const uint32_t *qinfo, uint32_t num_queries, const void *cands, uint32_t num_cands,
                          const uint8_t *bin_of_words, uint32_t max_words, uint32_t num_variants,
                          uint32_t *block_counts, uint32_t *ranges, uint32_t *order, uint32_t *err,

@@ -39,6 +39,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "myers_kernels.h"
@@ -1400,6 +1401,36 @@ extern "C" int myers_launch_long(const myers_verify_args_t *args, const myers_lo
     void *kargs[] = {const_cast<myers_verify_args_t *>(args), const_cast<myers_long_plan_t *>(plan)};
     return (int)hipLaunchKernel(reinterpret_cast<const void *>(&verify_long_kernel), dim3(grid),
                                 dim3(64u * MYERS_LONG_WAVES), kargs, lds, stream);
+}
+
+// ------------------------------------------------- static device workspace
+// The allocation-free entry point's workspace.  In host code the name
+// `scratch` stands for the variable, not a device address: the address on
+// each GPU comes from hipGetSymbolAddress, resolved once per device and kept.
+static __device__ __align__(256) uint32_t scratch[MYERS_SCRATCH_WORDS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t *g_scratch_ptr[kMaxDevices];
+}  // namespace
+
+extern "C" int myers_scratch_init(int device, uint32_t **ptr) {
+    if (device < 0 || device >= kMaxDevices || ptr == nullptr) return (int)hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device] == nullptr) {
+        int current = -1;
+        hipError_t e = hipGetDevice(&current);
+        if (e != hipSuccess) return (int)e;
+        if (current != device) return (int)hipErrorInvalidDevice;
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        if (p == nullptr) return (int)hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = static_cast<uint32_t *>(p);
+    }
+    *ptr = g_scratch_ptr[device];
+    return 0;
 }
 
 extern "C" int myers_launch_binning(const uint32_t *qinfo, uint32_t num_queries, const void *cands,

@@ -31,6 +31,7 @@ ENGINE_AUTO = 0
 ENGINE_PLAIN = 1 << 4
 ENGINE_PERSISTENT = 2 << 4
 USE_GRAPH = 1 << 6
+NOALLOC_MAX_CANDIDATES = 1 << 21     # myersVerifyDeviceNoAlloc's static workspace
 
 ERRORS = {
     0: "SUCCESS", 1: "E_OPENING_FILE", 2: "E_READING_FILE", 3: "E_INSUFFICIENT_MEM_GPU",
@@ -158,6 +159,7 @@ def lib():
         L.myersGetReferenceSize.argtypes = [vp]
         L.myersGetReferenceSize.restype = u64
         L.myersVerifyDevice.argtypes = [vp, vp, u32, vp, u32, vp, u32, f32, u32, vp, vp]
+        L.myersVerifyDeviceNoAlloc.argtypes = [vp, vp, u32, vp, u32, vp, u32, f32, u32, vp, vp]
         L.myersSynchronize.argtypes = [vp]
         L.myersGetStream.argtypes = [vp]
         L.myersGetStream.restype = vp

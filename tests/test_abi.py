@@ -27,6 +27,7 @@ def declared_functions(header):
     text = open(os.path.join(INCLUDE, header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     text = re.sub(r"//.*", "", text)
+    text = re.sub(r"^\s*#.*$", "", text, flags=re.M)       # preprocessor lines
     names = set()
     for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(([^;{}]*)\)\s*;", text):
         name = m.group(1)

@@ -1,17 +1,26 @@
 #!/usr/bin/env python3
 """Throughput of the candidate-verification path on one MI355X.
 
-Default workload (the headline configuration): seed-7 synthetic reference of
-3.1 Gbases (1.55 GB packed, resident in HBM), 100K reads of 1000 bp (3%
-substitutions, 0.5% insertions, 0.5% deletions, indel length geometric
-p=0.7), 11 candidates per read (1 true locus + 10 uniform), window 1200
-(distance 0.1), aligned windows (the API kernel's semantics), plain
-one-length-per-launch kernel.
+Headline workload (config 3, the mixed-length batch): seed-7 synthetic
+reference of 3.1 Gbases (1.55 GB packed, resident in HBM), 1M reads with
+lengths uniform on {100, 200, 400, 600, 800, 1000} bp, error uniform in
+[1%, 5%] (80% substitutions, 20% indels), Poisson(11) candidates per read
+capped at 64 with 1 true locus, window 1.2 m (distance 0.1), aligned windows
+(the API kernel's semantics), persistent engine (one queue launch over all
+length bins).  It is compared on the same batch with the plain engine (one
+launch per length bin, in turn), per-bin resident grids, and one launch of
+the kernel sized for the longest read.
 
-A step is one pass of the verification kernel over the whole batch, with
-the inputs already in HBM.  The script prints ONE JSON line.
+Beside the headline, by default: config 2 (100K x 1000 bp, plain one-length
+kernel) and config 4 (50K long noisy reads, 2-20 kb), each with its own
+roofline and a CPU check on a sample; threshold mode on config 2; the
+serving sweep (config 5).
 
-  python bench.py [--steps K] [--warmup W] [--config 1|2|3|4] [--engine plain|persistent]
+A step is one pass of the pipeline (binning + verification) over the whole
+batch, with the inputs already in HBM.  The script prints ONE JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--config 1|2|3|4] [--legs 2,4]
+                  [--engine auto|plain|persistent|binned]
 """
 import argparse
 import json
@@ -35,16 +44,26 @@ import numpy as np  # noqa: E402
 
 import myers_amd as M  # noqa: E402
 
-# MI355X published figures: 256 CUs, 2.4 GHz peak engine clock, 8 TB/s HBM3E.
-CUS, CLOCK_HZ, LANES_PER_CU_CLK = 256, 2.4e9, 64
-PEAK_INT_OPS = CUS * LANES_PER_CU_CLK * CLOCK_HZ            # 3.93e13 int32 lane-ops/s
+# MI355X published figures: 256 CUs, 2.4 GHz peak engine clock, 8 TB/s HBM3E,
+# 157.3 TFLOP/s FP32 vector = 256 CUs x 128 lanes x 2 flops (FMA) x 2.4 GHz.
+CUS, CLOCK_HZ = 256, 2.4e9
+# VALU issue peak: each SIMD issues a wave64 VALU instruction per quad-cycle,
+# two of the double-rate kinds (and/or/xor/bitop3/add without carry), so a
+# CU retires at most 128 lane-ops per clock: 78.6e12 lane-ops/s.
+PEAK_VALU_OPS = CUS * 128 * CLOCK_HZ
+PEAK_SINGLE_ISSUE_OPS = CUS * 64 * CLOCK_HZ                 # one instruction per quad-cycle
 PEAK_HBM_GBS = 8000.0
 # Algorithmic VALU ops per 32 cells (one u32 word of one column):
 #   K_GFX950 = 10: and, add-with-carry, 3 three-input logic ops (Xh, Ph, Pv),
 #   Mh and, Mv and, Xv or, 2 funnel shifts -- the recurrence's minimum on gfx950.
+#   7 of them issue at the double rate, the carry add and the two funnel
+#   shifts at the single rate: 6.5 single-issue slots per word, which caps
+#   this op mix at 10/6.5 x PEAK_SINGLE_ISSUE_OPS = 60.5e12 ops/s (194 TCUPS).
 #   K_REF = 18: the reference's u32 formulation (src/myers-4bits.c:117-134).
 K_GFX950 = 10
 K_REF = 18
+OP_MIX_SLOTS = 7 * 0.5 + 3 * 1.0
+PEAK_OP_MIX_OPS = PEAK_SINGLE_ISSUE_OPS * K_GFX950 / OP_MIX_SLOTS
 
 CONFIGS = {
     1: dict(name="config1", ref_seed=42, ref_size=4_000_000, reads=10_000, spec=dict(
@@ -57,9 +76,8 @@ CONFIGS = {
         log_uniform=(2000, 20000), err_range=(0.12, 0.12), sub_frac=0.5, candidates=4)),
 }
 DISTANCE = 0.1
-# HBM bytes per launch of the default workload (config 2, aligned windows),
-# from FETCH_SIZE and WRITE_SIZE counter runs scaled by a 1 GiB copy
-# calibration (profiles/hbm_traffic_verify_L1W32_config2.md).
+# HBM bytes per launch of the dominant kernel, from FETCH_SIZE and WRITE_SIZE
+# counter runs scaled by a 1 GiB copy calibration (profiles/hbm_traffic_*.md).
 MEASURED_TRAFFIC = {(2, "aligned"): 1423.9e6}
 
 
@@ -249,6 +267,7 @@ def run_steps(m, bufs, b, flags, steps, warmup):
     if st.batchesTotal != steps:
         raise SystemExit(f"bench: {st.batchesTotal} timed batches for {steps} steps")
     return {"elapsed": t1 - t0, "kernel_ms": st.kernelMsTotal / st.batchesTotal,
+            "verify_ms": st.verifyMsTotal / st.batchesTotal,
             "launches": st.kernelLaunchesTotal / st.batchesTotal, "binned": bool(st.binned),
             "cand_per_s": nc * steps / (t1 - t0)}
 
@@ -377,28 +396,172 @@ def measure_serving(m, ref, ref_size, loads, seconds, batch_cells=0, engine=0, s
     return out
 
 
+def roofline(cells, kernel_ms, hbm_bytes, traffic, kernel):
+    """VALU roofline of one launch: algorithmic ops (K_GFX950 per word-column
+    of the nominal cells) over the verification launches' HIP-event time."""
+    kernel_s = kernel_ms / 1e3
+    achieved = cells * K_GFX950 / 32.0 / kernel_s
+    hbm = hbm_bytes / kernel_s / 1e9
+    return {
+        "bound": "valu",
+        "achieved": achieved / 1e12,
+        "peak": PEAK_VALU_OPS / 1e12,
+        "unit": "TOP/s",
+        "frac": achieved / PEAK_VALU_OPS,
+        "traffic": traffic,
+        "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+        "kernel": kernel,
+        "kernel_ms": kernel_ms,
+        "ops_per_cell": K_GFX950 / 32.0,
+        "peak_note": "256 CUs x 128 lane-ops/clk x 2.4 GHz: the dual-issue VALU rate",
+        "op_mix_ceiling": {"peak": PEAK_OP_MIX_OPS / 1e12, "unit": "TOP/s", "frac": achieved / PEAK_OP_MIX_OPS,
+                           "gcups": PEAK_OP_MIX_OPS / (K_GFX950 / 32.0) / 1e9,
+                           "note": "7 double-rate + 3 single-rate ops per word: 6.5 single-issue slots"},
+        "ratio_to_single_issue_peak": achieved / PEAK_SINGLE_ISSUE_OPS,
+        "ratio_reference_ops_to_peak": cells * K_REF / 32.0 / kernel_s / PEAK_VALU_OPS,
+        "hbm": {"achieved": hbm, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": hbm / PEAK_HBM_GBS,
+                "algorithmic_bytes": hbm_bytes},
+    }
+
+
+def variant_names(m, b):
+    info = m.variants()
+    used = sorted({int(v) for v in m.bin_table()[np.unique((b["qinfo"][:, 1] + 31) // 32)]})
+    return [f"L{info[v]['lanes']}xW{info[v]['words']}" for v in used]
+
+
+def dominant_kernel(m, b, engine, run):
+    """Name of the launch the roofline is taken over."""
+    names = variant_names(m, b)
+    if not run["binned"]:
+        return f"verify_kernel<{names[0][1:].replace('xW', ', ')}, false> (one launch)"
+    if engine in (M.ENGINE_PERSISTENT, M.ENGINE_AUTO) and run["launches"] == 1:
+        return "verify_queue_kernel (one launch over every bin: " + ", ".join(names) + ")"
+    return f"{run['launches']:.0f} launches ({', '.join(names)})"
+
+
+def measure_config(m, ref, cfg_id, steps, warmup, engine, mode, graph, compare, cpu_seconds, threshold=-1.0):
+    """Time one configuration on the device-resident batch; check a sample
+    against the CPU; optionally compare the engines and time threshold mode."""
+    cfg = CONFIGS[cfg_id]
+    spec = M.make_spec(cfg["ref_seed"] + 1, cfg["reads"], distance=DISTANCE, **cfg["spec"])
+    t0 = time.time()
+    b = M.synth_batch(spec, ref, cfg["ref_size"])
+    nc, nq = len(b["cands"]), len(b["qinfo"])
+    log(f"{cfg['name']}: {nq} reads, {nc} candidates, {b['cells']:.3e} cells in {time.time() - t0:.1f}s")
+    mode_flag = M.WINDOW_EXACT if mode == "exact" else M.WINDOW_ALIGNED
+    gflag = M.USE_GRAPH if graph else 0
+    flags = mode_flag | engine | gflag
+    bufs = (M.DeviceArray(b["peq"]), M.DeviceArray(b["qinfo"]), M.DeviceArray(b["cands"]),
+            M.DeviceArray.empty((nc, 2), np.uint32))
+    try:
+        run = run_steps(m, bufs, b, flags, steps, warmup)
+        results = bufs[3].get()
+        cells = b["cells"]
+        value = nc * steps / run["elapsed"]
+        out = {
+            "workload": f"{cfg['name']}: seed-{cfg['ref_seed']} {cfg['ref_size'] / 1e9:.3g} Gbase reference, "
+                        f"{nq} reads, {nc} candidates, window = len*1.2, {mode} windows",
+            "candidates_per_s": value, "gcups": cells * steps / run["elapsed"] / 1e9,
+            "ms_per_step": run["elapsed"] / steps * 1e3, "steps": steps, "warmup": warmup,
+            "reads": nq, "candidates": nc, "cells": cells, "variants": variant_names(m, b),
+            "launches_per_step": run["launches"], "binned": run["binned"],
+            "pipeline_ms": run["kernel_ms"], "verify_ms": run["verify_ms"],
+            "roofline": roofline(cells, run["verify_ms"], algorithmic_bytes(b["qinfo"], b["cands"]),
+                                 MEASURED_TRAFFIC.get((cfg_id, mode)), dominant_kernel(m, b, engine, run)),
+        }
+        log(f"{cfg['name']}: {value:.4e} candidates/s, {out['gcups']:.0f} GCUPS, verify {run['verify_ms']:.3f} ms, "
+            f"pipeline {run['kernel_ms']:.3f} ms, {run['launches']:.0f} launches, "
+            f"frac {out['roofline']['frac']:.3f}")
+        if compare:
+            other = {}
+            for name, eng in (("plain", M.ENGINE_PLAIN), ("binned", M.ENGINE_BINNED),
+                              ("persistent", M.ENGINE_PERSISTENT)):
+                r = run_steps(m, bufs, b, mode_flag | eng | gflag, max(3, steps // 2), 1)
+                if not np.array_equal(bufs[3].get(), results):
+                    raise SystemExit(f"bench: engine {name} differs from the timed run")
+                other[name] = r
+            # the plain kernel sized for the longest read, one launch over the whole batch
+            table = m.bin_table()
+            wmax = int((int(b["qinfo"][:, 1].max()) + 31) // 32)
+            v_long = int(table[wmax])
+            forced = table.copy()
+            forced[1:wmax + 1] = v_long
+            m.set_bin_table(forced)
+            one = run_steps(m, bufs, b, mode_flag | M.ENGINE_PLAIN, max(3, steps // 2), 1)
+            if not np.array_equal(bufs[3].get(), results):
+                raise SystemExit("bench: the one-variant launch differs from the timed run")
+            m.set_bin_table(table)
+            vinfo = m.variants()[v_long]
+            cps = {k: r["cand_per_s"] for k, r in other.items()}
+            out["engine_comparison"] = {
+                "plain_candidates_per_s": cps["plain"], "binned_candidates_per_s": cps["binned"],
+                "persistent_candidates_per_s": cps["persistent"],
+                "plain_verify_ms": other["plain"]["verify_ms"], "binned_verify_ms": other["binned"]["verify_ms"],
+                "persistent_verify_ms": other["persistent"]["verify_ms"],
+                "plain_launches": other["plain"]["launches"], "binned_launches": other["binned"]["launches"],
+                "persistent_launches": other["persistent"]["launches"],
+                "persistent_over_plain": cps["persistent"] / cps["plain"],
+                "persistent_over_binned": cps["persistent"] / cps["binned"],
+                "one_launch_variant": f"L{vinfo['lanes']}xW{vinfo['words']}",
+                "one_launch_candidates_per_s": one["cand_per_s"],
+                "persistent_over_one_launch": cps["persistent"] / one["cand_per_s"],
+                "note": "plain = one launch per length bin, in turn, on one stream; binned = one resident grid "
+                        "per bin on forked streams; persistent = one queue launch over every bin; one launch = "
+                        "the kernel for the longest read over every candidate",
+            }
+            log(f"{cfg['name']} engines: " + ", ".join(f"{k} {v:.4e}" for k, v in cps.items()) +
+                f", one launch {one['cand_per_s']:.4e}")
+        if threshold >= 0:
+            m.set_threshold(threshold)
+            r = run_steps(m, bufs, b, flags, max(3, steps // 2), 1)
+            m.set_threshold(-1.0)
+            tres = bufs[3].get()
+            # same results as the full run for candidates under the threshold, the no-match pair otherwise
+            t = np.array([threshold_of(x, threshold) for x in range(int(b["qinfo"][:, 1].max()) + 1)],
+                         dtype=np.int64)[b["qinfo"][b["cands"]["query"], 1]]
+            under = (results[:, 1] != 0xFFFFFFFF) & (results[:, 1].astype(np.int64) <= t)
+            skipped = results[:, 1] == 0xFFFFFFFF
+            ok = (np.array_equal(tres[under], results[under]) and np.array_equal(tres[skipped], results[skipped])
+                  and np.all(tres[~under & ~skipped] == 0xFFFFFFFF))
+            if not ok:
+                raise SystemExit("bench: threshold-mode results differ from the full run")
+            out["threshold_mode"] = {"error_rate": threshold, "candidates_per_s": r["cand_per_s"],
+                                     "verify_ms": r["verify_ms"], "under_threshold": int(under.sum()),
+                                     "speedup_vs_full": r["cand_per_s"] / value}
+            log(f"{cfg['name']} threshold mode: {out['threshold_mode']}")
+        if cpu_seconds > 0:
+            log(f"{cfg['name']}: CPU baseline / parity sample ...")
+            out["cpu_baseline"] = cpu_baseline(ref, cfg["ref_size"], b, results, cpu_seconds, mode == "exact")
+            log(f"{cfg['name']}: CPU {out['cpu_baseline']['value']:.4e} candidates/s "
+                f"({out['cpu_baseline']['kind']})")
+        return out
+    finally:
+        for a in bufs:
+            a.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
-    ap.add_argument("--engine", default="auto", choices=["auto", "plain", "persistent"])
+    ap.add_argument("--config", type=int, default=3, choices=sorted(CONFIGS), help="headline configuration")
+    ap.add_argument("--legs", default="2,4", help="other configurations measured beside the headline ('' = none)")
+    ap.add_argument("--engine", default="auto", choices=["auto", "plain", "persistent", "binned"])
     ap.add_argument("--mode", default="aligned", choices=["aligned", "exact"])
     ap.add_argument("--graph", action="store_true", help="replay the batch pipeline as a captured HIP graph")
-    ap.add_argument("--compare-engines", action="store_true",
-                    help="also time the plain and persistent engines, and one plain launch of the variant "
-                         "for the longest read, on the same batch")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
+    ap.add_argument("--no-compare", action="store_true", help="skip the headline's engine comparison")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget per config (0 = skip)")
     ap.add_argument("--serve-loads", default="0.1,0.3,0.5,0.7,1.0",
                     help="serving load levels (fractions of saturation); '' = skip")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--serve-buffers", type=int, default=3)
     ap.add_argument("--serve-batch-cells", type=float, default=0, help="serving batch cap on cells (0 = library default)")
-    ap.add_argument("--serve-engine", default="auto", choices=["auto", "plain", "persistent"])
+    ap.add_argument("--serve-engine", default="auto", choices=["auto", "plain", "persistent", "binned"])
     ap.add_argument("--variant", default=None, help="force a kernel variant for every read length, e.g. L2xW16")
     ap.add_argument("--threshold", type=float, default=0.2,
-                    help="also time threshold mode at this error rate (negative = skip)")
+                    help="also time threshold mode at this error rate on config 2 (negative = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (candidates would shard by read "
@@ -406,24 +569,21 @@ def main():
         sys.exit(2)
 
     cfg = CONFIGS[args.config]
+    legs = [int(x) for x in args.legs.split(",") if x.strip() and int(x) != args.config]
+    legs = [c for c in legs if CONFIGS[c]["ref_seed"] == cfg["ref_seed"] and CONFIGS[c]["ref_size"] == cfg["ref_size"]]
     t0 = time.time()
     ref = M.synth_reference(cfg["ref_seed"], cfg["ref_size"])
     log(f"reference {cfg['ref_size']} bases in {time.time() - t0:.1f}s")
-    spec = M.make_spec(cfg["ref_seed"] + 1, cfg["reads"], distance=DISTANCE, **cfg["spec"])
-    t0 = time.time()
-    b = M.synth_batch(spec, ref, cfg["ref_size"])
-    nc, nq = len(b["cands"]), len(b["qinfo"])
-    log(f"batch: {nq} reads, {nc} candidates, {b['cells']:.3e} cells in {time.time() - t0:.1f}s")
 
-    engines = {"auto": M.ENGINE_AUTO, "plain": M.ENGINE_PLAIN, "persistent": M.ENGINE_PERSISTENT}
-    mode_flag = M.WINDOW_EXACT if args.mode == "exact" else M.WINDOW_ALIGNED
-    flags = mode_flag | engines[args.engine] | (M.USE_GRAPH if args.graph else 0)
-    avg = int(b["qinfo"][:, 1].mean())
+    engines = {"auto": M.ENGINE_AUTO, "plain": M.ENGINE_PLAIN, "persistent": M.ENGINE_PERSISTENT,
+               "binned": M.ENGINE_BINNED}
     serve_loads = [float(x) for x in args.serve_loads.split(",") if x.strip()]
     num_buffers = args.serve_buffers if serve_loads else 1
-    need = 2 * (b["peq"].nbytes + b["cands"].nbytes + nc * 12)
-    m = M.Myers(ref, cfg["ref_size"], num_buffers=num_buffers, avg_query=avg, cands_per_query=max(1, nc // nq),
-                buffer_bytes=max(512 << 20, need))
+    # buffers sized for the largest batch measured (config 3: 11M candidates)
+    sizes = {c: CONFIGS[c]["reads"] * 12 for c in [args.config] + legs}
+    max_cands = max(sizes.values())
+    m = M.Myers(ref, cfg["ref_size"], num_buffers=num_buffers, avg_query=550, cands_per_query=11,
+                buffer_bytes=max(512 << 20, int(max_cands * 130)))
     if args.variant:
         lanes, words = (int(x) for x in args.variant.lstrip("L").split("xW"))
         v = m.variant_index(lanes, words)
@@ -431,79 +591,17 @@ def main():
         w = np.arange(len(table))
         table[(w >= 1) & (w <= lanes * words)] = v
         m.set_bin_table(table)
-    bufs = (M.DeviceArray(b["peq"]), M.DeviceArray(b["qinfo"]), M.DeviceArray(b["cands"]),
-            M.DeviceArray.empty((nc, 2), np.uint32))
 
-    run = run_steps(m, bufs, b, flags, args.steps, args.warmup)
-    results = bufs[3].get()
-    comparison = None
-    if args.compare_engines:
-        other = {}
-        for name in ("plain", "persistent"):
-            r = run_steps(m, bufs, b, mode_flag | engines[name] | (M.USE_GRAPH if args.graph else 0),
-                          max(3, args.steps // 2), 1)
-            if not np.array_equal(bufs[3].get(), results):
-                raise SystemExit(f"bench: engine {name} differs from the timed run")
-            other[name] = r
-        # the plain kernel sized for the longest read, one launch over the whole batch
-        table = m.bin_table()
-        wmax = int((int(b["qinfo"][:, 1].max()) + 31) // 32)
-        v_long = int(table[wmax])
-        forced = table.copy()
-        forced[1:wmax + 1] = v_long
-        m.set_bin_table(forced)
-        one = run_steps(m, bufs, b, mode_flag | engines["plain"], max(3, args.steps // 2), 1)
-        if not np.array_equal(bufs[3].get(), results):
-            raise SystemExit("bench: the one-variant launch differs from the timed run")
-        m.set_bin_table(table)
-        vinfo = m.variants()[v_long]
-        comparison = {"plain_candidates_per_s": other["plain"]["cand_per_s"],
-                      "persistent_candidates_per_s": other["persistent"]["cand_per_s"],
-                      "plain_kernel_ms": other["plain"]["kernel_ms"],
-                      "persistent_kernel_ms": other["persistent"]["kernel_ms"],
-                      "persistent_over_plain": other["persistent"]["cand_per_s"] / other["plain"]["cand_per_s"],
-                      "one_launch_variant": f"L{vinfo['lanes']}xW{vinfo['words']}",
-                      "one_launch_candidates_per_s": one["cand_per_s"],
-                      "persistent_over_one_launch": other["persistent"]["cand_per_s"] / one["cand_per_s"]}
-        log(f"engines: {comparison}")
-
-    elapsed, kernel_ms = run["elapsed"], run["kernel_ms"]
-    value = nc * args.steps / elapsed
-    cells = b["cells"]
-    gcups = cells * args.steps / elapsed / 1e9
-    kernel_s = kernel_ms / 1e3
-    valu_achieved = cells * K_GFX950 / 32.0 / kernel_s / 1e12
-    hbm_bytes = algorithmic_bytes(b["qinfo"], b["cands"])
-    hbm_achieved = hbm_bytes / kernel_s / 1e9
-
-    info = m.variants()
-    used = sorted({int(v) for v in m.bin_table()[np.unique((b["qinfo"][:, 1] + 31) // 32)]})
-    variants = [f"L{info[v]['lanes']}xW{info[v]['words']}" for v in used]
-
-    threshold = None
-    if args.threshold >= 0:
-        m.set_threshold(args.threshold)
-        r = run_steps(m, bufs, b, flags, max(3, args.steps // 2), 1)
-        m.set_threshold(-1.0)
-        tres = bufs[3].get()
-        # same results as the full run for candidates under the threshold, the no-match pair otherwise
-        t = np.array([threshold_of(x, args.threshold) for x in range(int(b["qinfo"][:, 1].max()) + 1)],
-                     dtype=np.int64)[b["qinfo"][b["cands"]["query"], 1]]
-        under = (results[:, 1] != 0xFFFFFFFF) & (results[:, 1].astype(np.int64) <= t)
-        skipped = results[:, 1] == 0xFFFFFFFF
-        ok = (np.array_equal(tres[under], results[under]) and np.array_equal(tres[skipped], results[skipped])
-              and np.all(tres[~under & ~skipped] == 0xFFFFFFFF))
-        if not ok:
-            raise SystemExit("bench: threshold-mode results differ from the full run")
-        threshold = {"error_rate": args.threshold, "candidates_per_s": r["cand_per_s"],
-                     "kernel_ms": r["kernel_ms"], "under_threshold": int(under.sum()),
-                     "speedup_vs_full": r["cand_per_s"] / (nc * args.steps / run["elapsed"])}
-        log(f"threshold mode: {threshold}")
-
-    cpu = None
-    if args.cpu_seconds > 0:
-        log("CPU baseline / parity sample ...")
-        cpu = cpu_baseline(ref, cfg["ref_size"], b, results, args.cpu_seconds, args.mode == "exact")
+    head = measure_config(m, ref, args.config, args.steps, args.warmup, engines[args.engine], args.mode, args.graph,
+                          not args.no_compare, args.cpu_seconds,
+                          args.threshold if args.config == 2 else -1.0)
+    leg_out = {}
+    for c in legs:
+        steps = args.steps if c == 2 else max(3, args.steps // 4)
+        leg_out[CONFIGS[c]["name"]] = measure_config(
+            m, ref, c, steps, max(1, args.warmup // 2), engines[args.engine], args.mode, args.graph,
+            c == 4 and not args.no_compare, args.cpu_seconds / 2 if c != 4 else args.cpu_seconds / 2,
+            args.threshold if c == 2 else -1.0)
 
     serving = None
     if serve_loads:
@@ -513,48 +611,34 @@ def main():
 
     out = {
         "metric": "candidates/s & GCUPS on 1 MI355X (bit-exact); p99 latency under serving load",
-        "value": value,
+        "value": head["candidates_per_s"],
         "unit": "candidates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (seeded generator, see include/myers-tools.h)",
-        "gcups": gcups,
+        "gcups": head["gcups"],
         "config": {
-            "workload": f"{cfg['name']}: seed-{cfg['ref_seed']} {cfg['ref_size'] / 1e9:.3g} Gbase reference, "
-                        f"{nq} reads, {nc} candidates, window = len*1.2, {args.mode} windows",
-            "reads": nq, "candidates": nc, "cells": cells, "reference_bases": cfg["ref_size"],
-            "engine": args.engine, "graph": args.graph, "variants": variants,
-            "launches_per_step": run["launches"], "binned": run["binned"],
+            "workload": head["workload"],
+            "reads": head["reads"], "candidates": head["candidates"], "cells": head["cells"],
+            "reference_bases": cfg["ref_size"], "engine": args.engine, "graph": args.graph,
+            "variants": head["variants"], "launches_per_step": head["launches_per_step"], "binned": head["binned"],
+            "pipeline_ms": head["pipeline_ms"], "verify_ms": head["verify_ms"],
         },
-        "roofline": {
-            "bound": "valu",
-            "achieved": valu_achieved,
-            "peak": PEAK_INT_OPS / 1e12,
-            "unit": "TOP/s",
-            "frac": valu_achieved / (PEAK_INT_OPS / 1e12),
-            "traffic": MEASURED_TRAFFIC.get((args.config, args.mode)),
-            "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-            "kernel_ms": kernel_ms,
-            "kernel_launches_per_step": run["launches"],
-            "ops_per_cell": K_GFX950 / 32.0,
-            "frac_reference_ops": cells * K_REF / 32.0 / kernel_s / PEAK_INT_OPS,
-            "hbm": {"achieved": hbm_achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": hbm_achieved / PEAK_HBM_GBS, "algorithmic_bytes": hbm_bytes},
-        },
-        "engine_comparison": comparison,
-        "threshold_mode": threshold,
-        "cpu_baseline": cpu,
+        "roofline": head["roofline"],
+        "engine_comparison": head.get("engine_comparison"),
+        "threshold_mode": head.get("threshold_mode") or next(
+            (x["threshold_mode"] for x in leg_out.values() if "threshold_mode" in x), None),
+        "cpu_baseline": head.get("cpu_baseline"),
+        "legs": leg_out,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)
-    for a in bufs:
-        a.free()
     m.close()
 
 

@@ -48,11 +48,17 @@ typedef enum {
 /* Engine selection (extension). */
 #define MYERS_ENGINE_AUTO       0u
 #define MYERS_ENGINE_PLAIN      (1u << 4)  /* one launch per length bin */
-#define MYERS_ENGINE_PERSISTENT (2u << 4)  /* one persistent launch over a length-binned queue */
+#define MYERS_ENGINE_PERSISTENT (2u << 4)  /* one persistent launch over a length-binned device queue
+                                              (per kernel class: one-lane / multi-lane variants) */
+#define MYERS_ENGINE_BINNED     (3u << 4)  /* one resident grid per length bin, bins on forked streams */
 #define MYERS_ENGINE_MASK       (3u << 4)
 /* Capture the batch's on-device pipeline (binning + kernels) into a HIP
  * graph once per batch shape and replay it (extension). */
 #define MYERS_USE_GRAPH         (1u << 6)
+/* Persistent engine: do not give the long-read bins of a small batch their
+ * own latency-ordered launch; every bin goes into the queue launches
+ * (extension; throughput scheduling only). */
+#define MYERS_NO_LONG_LAUNCH    (1u << 7)
 
 /* Longest query supported, in bases (extension: the reference stops at 4096,
  * src-api/myersGPU.c:366,406-410). */
@@ -84,6 +90,10 @@ typedef struct {
     uint64_t kernelLaunchesTotal;   /* verification launches */
     uint64_t batchesTotal;          /* timed batches */
     uint64_t graphCaptures;         /* graphs captured (MYERS_USE_GRAPH) */
+    /* the verification launches alone (from the end of the device-side
+     * binning; graph replays: the whole pipeline, as kernelMs) */
+    double   verifyMs;              /* last timed batch */
+    double   verifyMsTotal;         /* accumulated as kernelMsTotal */
 } myersStats_t;
 
 void myersDefaultConfig(myersConfig_t *cfg);

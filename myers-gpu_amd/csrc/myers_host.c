@@ -58,6 +58,8 @@ typedef struct myers_ctx {
     int       occupancy_p[MYERS_MAX_VARIANTS];     /* persistent kernel */
     uint32_t  chunk_factor[MYERS_MAX_VARIANTS];    /* persistent: chunks per queue pull */
     uint32_t  long_seq;                            /* long-read launches so far (placement rotation) */
+    int       queue_occ[MYERS_QUEUE_CLASSES][2];   /* queue kernel workgroups per CU, [class][cut], */
+    uint32_t  queue_occ_lds[MYERS_QUEUE_CLASSES][2];  /* for this dynamic LDS size (0: not queried) */
     uint8_t   bin_of_words[MAX_QUERY_WORDS + 1];
     uint8_t  *d_bin_of_words;
     uint64_t  tune_gen;          /* bumped when the bin table or chunk factors change */
@@ -87,10 +89,12 @@ typedef struct myers_buffer {
     /* HIP events around the verification launches of each batch, in a ring
      * so that batches enqueued back to back are all timed; folded into the
      * stats in order when the stream is synchronised. */
-    hipEvent_t   ev_kernel[EV_RING][2];
+    hipEvent_t   ev_kernel[EV_RING][3];   /* start, end, start of the verification launches */
     uint32_t     ev_launches[EV_RING];
+    uint8_t      ev_marked[EV_RING];      /* event 2 was recorded (eager batches after binning) */
+    int          mark_slot;               /* slot whose event 2 enqueue_pipeline records, or -1 */
     uint32_t     ev_head, ev_pending;
-    /* persistent engine: variants run on forked streams joined by events */
+    /* launches of one batch that run side by side: forked streams joined by events */
     hipStream_t  side[SIDE_STREAMS];
     hipEvent_t   ev_fork, ev_join[SIDE_STREAMS];
     /* steady-state graph of the last device-resident batch shape */
@@ -293,8 +297,8 @@ static void free_buffer(myers_buffer_t *b)
     if (b->ev_batch_start) hipEventDestroy(b->ev_batch_start);
     if (b->ev_batch_end) hipEventDestroy(b->ev_batch_end);
     for (uint32_t i = 0; i < EV_RING; i++) {
-        if (b->ev_kernel[i][0]) hipEventDestroy(b->ev_kernel[i][0]);
-        if (b->ev_kernel[i][1]) hipEventDestroy(b->ev_kernel[i][1]);
+        for (uint32_t k = 0; k < 3; k++)
+            if (b->ev_kernel[i][k]) hipEventDestroy(b->ev_kernel[i][k]);
     }
     if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
     for (uint32_t i = 0; i < SIDE_STREAMS; i++) {
@@ -332,12 +336,12 @@ static int init_buffer(myers_buffer_t *b)
     HIP_TRY(hipEventCreate(&b->ev_batch_start));
     HIP_TRY(hipEventCreate(&b->ev_batch_end));
     for (uint32_t i = 0; i < EV_RING; i++) {
-        HIP_TRY(hipEventCreate(&b->ev_kernel[i][0]));
-        HIP_TRY(hipEventCreate(&b->ev_kernel[i][1]));
+        for (uint32_t k = 0; k < 3; k++) HIP_TRY(hipEventCreate(&b->ev_kernel[i][k]));
     }
     memset(b->h_results, 0, re_b);
     *b->h_err = 0;
     b->stats.structSize = sizeof(myersStats_t);
+    b->mark_slot = -1;
     return MYERS_SUCCESS;
 }
 
@@ -572,10 +576,15 @@ static int fold_oldest_timing(myers_buffer_t *b)
     const uint32_t slot = (b->ev_head + EV_RING - b->ev_pending) % EV_RING;
     int rc = hip_fail(hipEventSynchronize(b->ev_kernel[slot][1]), "hipEventSynchronize", __FILE__, __LINE__);
     if (rc) return rc;
-    float ms = 0.0f;
+    float ms = 0.0f, vms = 0.0f;
     if (hipEventElapsedTime(&ms, b->ev_kernel[slot][0], b->ev_kernel[slot][1]) == hipSuccess) {
+        if (!b->ev_marked[slot] ||
+            hipEventElapsedTime(&vms, b->ev_kernel[slot][2], b->ev_kernel[slot][1]) != hipSuccess)
+            vms = ms;
         b->stats.kernelMs = ms;
         b->stats.kernelMsTotal += ms;
+        b->stats.verifyMs = vms;
+        b->stats.verifyMsTotal += vms;
         b->stats.kernelLaunchesTotal += b->ev_launches[slot];
         b->stats.batchesTotal++;
     }
@@ -590,6 +599,7 @@ static int timing_begin(myers_buffer_t *b, uint32_t *slot)
         if (rc) return rc;
     }
     *slot = b->ev_head;
+    b->ev_marked[*slot] = 0;
     return hip_fail(hipEventRecord(b->ev_kernel[*slot][0], b->stream), "hipEventRecord", __FILE__, __LINE__);
 }
 
@@ -730,6 +740,43 @@ static uint32_t long_plan(const myers_ctx_t *ctx, const bin_scan_t *scan, const 
     return nt;
 }
 
+/* Resident one-wave workgroups per CU of a queue kernel with `lds` bytes of
+ * dynamic LDS (queried once per size). */
+static uint32_t queue_grid(myers_ctx_t *ctx, uint32_t cls, int cut, uint32_t lds)
+{
+    if (ctx->queue_occ_lds[cls][cut] != lds) {
+        int occ = 0;
+        if (myers_queue_occupancy(cls, cut, lds, &occ) != 0 || occ <= 0) occ = 1;
+        ctx->queue_occ[cls][cut] = occ;
+        ctx->queue_occ_lds[cls][cut] = lds;
+    }
+    return (uint32_t)ctx->num_cus * (uint32_t)ctx->queue_occ[cls][cut];
+}
+
+/* Queue launches of a persistent batch: the bins (already in longest-first
+ * order) of each kernel class go into one plan, in that order; bins of no
+ * class (multi-lane W = 16, multi-wave variants) are left in `other`. */
+static void queue_plans(const uint32_t *bins, uint32_t n, myers_queue_plan_t *qp, uint32_t *other,
+                        uint32_t *nother)
+{
+    memset(qp, 0, sizeof(myers_queue_plan_t) * MYERS_QUEUE_CLASSES);
+    *nother = 0;
+    for (uint32_t c = 0; c < (uint32_t)MYERS_QUEUE_CLASSES; c++) qp[c].cls = c;
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t u4 = 0;
+        const int cls = myers_variant_queue_class(bins[i], &u4);
+        if (cls < 0 || qp[cls].n >= (uint32_t)MYERS_QUEUE_MAX) {
+            other[(*nother)++] = bins[i];
+            continue;
+        }
+        myers_queue_plan_t *q = &qp[cls];
+        q->variant[q->n] = bins[i];
+        q->key[q->n] = 0;   /* filled by the caller from the variant table */
+        q->n++;
+        if (u4 > q->ws_u4) q->ws_u4 = u4;
+    }
+}
+
 /* Enqueue the whole on-device pipeline of one batch on b->stream (no timing
  * events, so that it can be captured into a graph). */
 static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan_t *scan)
@@ -776,6 +823,11 @@ static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan
                               b->stream);
     if (rc) return hip_fail((hipError_t)rc, "binning launch", __FILE__, __LINE__);
     HIP_TRY(hipMemcpyAsync(b->h_err, err, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
+    if (b->mark_slot >= 0) {
+        /* eager batches: the verification launches are timed apart from the binning */
+        HIP_TRY(hipEventRecord(b->ev_kernel[b->mark_slot][2], b->stream));
+        b->ev_marked[b->mark_slot] = 1;
+    }
     b->binned = 1;
     a.order = bt->d_order;
     a.bin_ranges = ranges;
@@ -791,12 +843,18 @@ static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan
             if (rc) return hip_fail((hipError_t)rc, "verify launch", __FILE__, __LINE__);
         }
     } else {
-        /* persistent: every bin's resident grid drains its queue; the bins run
-         * on forked streams so that one bin's tail overlaps the next bin.  The
-         * long-read bins of a latency-bound batch share one launch on the
-         * first side stream, the others go round-robin on the rest. */
+        /* Persistent engines.  The long-read bins of a latency-bound batch
+         * share one launch (long_plan).  MYERS_ENGINE_PERSISTENT puts the
+         * other bins into one queue launch per kernel class (one launch for
+         * a batch of one class); MYERS_ENGINE_BINNED gives every bin its own
+         * resident grid.  Bins without a queue class keep their own grid.
+         * When more than one launch results, they run on forked streams
+         * (long-read launch first), so that each one's tail overlaps the
+         * others. */
         myers_long_plan_t lp;
-        const uint32_t nlong = long_plan(ctx, scan, bt, order, n, &lp);
+        uint32_t nlong = 0;
+        if (bt->flags & MYERS_NO_LONG_LAUNCH) memset(&lp, 0, sizeof(lp));
+        else nlong = long_plan(ctx, scan, bt, order, n, &lp);
         uint32_t rest[MYERS_MAX_VARIANTS], nrest = 0;
         for (uint32_t i = 0; i < n; i++) {
             int in_long = 0;
@@ -804,29 +862,53 @@ static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan
             for (uint32_t j = 0; j < lp.n_wide; j++) in_long |= lp.wide_variant[j] == order[i];
             if (!in_long) rest[nrest++] = order[i];
         }
-        launches = nrest + (nlong ? 1u : 0u);
-        const uint32_t width = launches < SIDE_STREAMS ? launches : SIDE_STREAMS;
-        const uint32_t first = nlong ? 1u : 0u;
-        HIP_TRY(hipEventRecord(b->ev_fork, b->stream));
-        for (uint32_t j = 0; j < width; j++) HIP_TRY(hipStreamWaitEvent(b->side[j], b->ev_fork, 0));
+        myers_queue_plan_t qp[MYERS_QUEUE_CLASSES];
+        uint32_t other[MYERS_MAX_VARIANTS], nother = 0, nqueue = 0;
+        if (engine == MYERS_ENGINE_PERSISTENT) {
+            queue_plans(rest, nrest, qp, other, &nother);
+            for (uint32_t c = 0; c < (uint32_t)MYERS_QUEUE_CLASSES; c++) {
+                for (uint32_t j = 0; j < qp[c].n; j++)
+                    qp[c].key[j] = (ctx->variants[qp[c].variant[j]].lanes << 8) | ctx->variants[qp[c].variant[j]].words;
+                nqueue += qp[c].n ? 1u : 0u;
+            }
+        } else {
+            memset(qp, 0, sizeof(qp));
+            memcpy(other, rest, nrest * sizeof(uint32_t));
+            nother = nrest;
+        }
+        launches = (nlong ? 1u : 0u) + nqueue + nother;
+        const int fork = launches > 1u;
+        const uint32_t width = !fork ? 0u : (launches < SIDE_STREAMS ? launches : SIDE_STREAMS);
+        uint32_t next = 0;    /* launches placed so far: launch k goes on side stream k % width */
+#define NEXT_STREAM() (fork ? b->side[(next++) % width] : b->stream)
+        if (fork) {
+            HIP_TRY(hipEventRecord(b->ev_fork, b->stream));
+            for (uint32_t j = 0; j < width; j++) HIP_TRY(hipStreamWaitEvent(b->side[j], b->ev_fork, 0));
+        }
         if (nlong) {
             /* successive launches start their first items a golden-ratio step
              * of workgroups apart (overlapping batches use other CUs) */
             const uint32_t seq = __atomic_fetch_add(&ctx->long_seq, 1u, __ATOMIC_RELAXED);
             const uint32_t wg = (uint32_t)(((uint64_t)seq * 2654435769ull & 0xFFFFFFFFull) * ctx->num_cus >> 32);
             lp.rotate = wg * MYERS_LONG_WAVES;
-            rc = myers_launch_long(&a, &lp, ctx->num_cus, b->side[0]);
+            rc = myers_launch_long(&a, &lp, ctx->num_cus, NEXT_STREAM());
             if (rc) return hip_fail((hipError_t)rc, "long-read launch", __FILE__, __LINE__);
         }
-        for (uint32_t i = 0; i < nrest; i++) {
-            const uint32_t v = rest[i];
+        for (uint32_t c = 0; c < (uint32_t)MYERS_QUEUE_CLASSES; c++) {
+            if (qp[c].n == 0) continue;
+            const int cut = bt->threshold >= 0.0f;
+            rc = myers_launch_queue(&a, &qp[c], queue_grid(ctx, c, cut, qp[c].ws_u4 * 16u), NEXT_STREAM());
+            if (rc) return hip_fail((hipError_t)rc, "queue launch", __FILE__, __LINE__);
+        }
+        for (uint32_t i = 0; i < nother; i++) {
+            const uint32_t v = other[i];
             a.variant = v;
             a.chunk_factor = ctx->chunk_factor[v];
             uint64_t grid = (uint64_t)ctx->num_cus * (uint64_t)ctx->occupancy_p[v];
-            const uint32_t side = width > first ? first + i % (width - first) : 0u;
-            rc = myers_launch_verify(v, &a, (uint32_t)grid, 1, b->side[side]);
+            rc = myers_launch_verify(v, &a, (uint32_t)grid, 1, NEXT_STREAM());
             if (rc) return hip_fail((hipError_t)rc, "persistent launch", __FILE__, __LINE__);
         }
+#undef NEXT_STREAM
         for (uint32_t j = 0; j < width; j++) {
             HIP_TRY(hipEventRecord(b->ev_join[j], b->side[j]));
             HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_join[j], 0));
@@ -882,7 +964,10 @@ static int enqueue_verify(myers_buffer_t *b, const batch_t *bt, const qryInfo_t 
     int rc;
     if (!(bt->flags & MYERS_USE_GRAPH) || bt->num_cands == 0) {
         if ((rc = timing_begin(b, &slot))) return rc;
-        if ((rc = enqueue_pipeline(b, bt, &scan))) return rc;
+        b->mark_slot = (int)slot;
+        rc = enqueue_pipeline(b, bt, &scan);
+        b->mark_slot = -1;
+        if (rc) return rc;
         return timing_end(b, slot, b->stats.launches);
     }
     uint32_t dist_bits, thr_bits;
@@ -890,9 +975,10 @@ static int enqueue_verify(myers_buffer_t *b, const batch_t *bt, const qryInfo_t 
     memcpy(&thr_bits, &bt->threshold, sizeof(thr_bits));
     uint32_t order[MYERS_MAX_VARIANTS];
     myers_long_plan_t lp;
-    const uint64_t plan_key = (uint64_t)long_plan(b->ctx, &scan, bt, order,
-                                                  present_by_cost(b->ctx, scan.present, order), &lp) |
-                              ((uint64_t)lp.n_wide << 32);
+    const uint64_t plan_key = (bt->flags & MYERS_NO_LONG_LAUNCH) ? 0u :
+                              ((uint64_t)long_plan(b->ctx, &scan, bt, order,
+                                                   present_by_cost(b->ctx, scan.present, order), &lp) |
+                               ((uint64_t)lp.n_wide << 32));
     const uint64_t key[12] = {(uint64_t)(uintptr_t)bt->d_peq, (uint64_t)(uintptr_t)bt->d_qinfo,
                               (uint64_t)(uintptr_t)bt->d_cands, (uint64_t)(uintptr_t)bt->d_results,
                               ((uint64_t)bt->num_entries << 32) | bt->num_queries,
@@ -947,7 +1033,8 @@ int myersSendEx(void *myersBuffer, float distance, uint32_t numPEQEntries, uint3
 {
     myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
     if (b == NULL) return MYERS_E_INVALID_ARGUMENT;
-    if (flags & ~(1u | MYERS_ENGINE_MASK | MYERS_USE_GRAPH)) return (b->last_error = MYERS_E_INVALID_ARGUMENT);
+    if (flags & ~(1u | MYERS_ENGINE_MASK | MYERS_USE_GRAPH | MYERS_NO_LONG_LAUNCH))
+        return (b->last_error = MYERS_E_INVALID_ARGUMENT);
     if (numPEQEntries > b->max_peq_entries || numQueries > b->max_queries || numCandidates > b->max_candidates)
         return (b->last_error = MYERS_E_CAPACITY);
     int rc = hip_fail(hipSetDevice(b->ctx->device), "hipSetDevice", __FILE__, __LINE__);
@@ -1058,7 +1145,7 @@ int myersVerifyDevice(void *myersBuffer, const void *d_peq, uint32_t numPEQEntri
 {
     myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
     if (b == NULL) return MYERS_E_INVALID_ARGUMENT;
-    if ((flags & ~(1u | MYERS_ENGINE_MASK | MYERS_USE_GRAPH)) ||
+    if ((flags & ~(1u | MYERS_ENGINE_MASK | MYERS_USE_GRAPH | MYERS_NO_LONG_LAUNCH)) ||
         (numCandidates && (!d_peq || !d_qinfo || !d_candidates || !d_results)))
         return (b->last_error = MYERS_E_INVALID_ARGUMENT);
     if (numCandidates > b->max_candidates) return (b->last_error = MYERS_E_CAPACITY);
@@ -1077,7 +1164,7 @@ int myersVerifyDeviceNoAlloc(void *myersBuffer, const void *d_peq, uint32_t numP
     myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
     if (b == NULL) return MYERS_E_INVALID_ARGUMENT;
     /* graph instantiation may allocate: not on this path */
-    if ((flags & ~(1u | MYERS_ENGINE_MASK)) ||
+    if ((flags & ~(1u | MYERS_ENGINE_MASK | MYERS_NO_LONG_LAUNCH)) ||
         (numCandidates && (!d_peq || !d_qinfo || !d_candidates || !d_results)))
         return (b->last_error = MYERS_E_INVALID_ARGUMENT);
     if (numCandidates > MYERS_NOALLOC_MAX_CANDIDATES) {
@@ -1118,7 +1205,10 @@ int myersGetStats(void *myersBuffer, myersStats_t *out)
 {
     myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
     if (b == NULL || out == NULL) return MYERS_E_INVALID_ARGUMENT;
-    *out = b->stats;
+    /* callers built against an older, shorter struct get its fields only */
+    const size_t n = out->structSize && out->structSize < sizeof(*out) ? out->structSize : sizeof(*out);
+    memcpy(out, &b->stats, n);
+    out->structSize = (uint32_t)n;
     return MYERS_SUCCESS;
 }
 
@@ -1127,6 +1217,7 @@ void myersResetStats(void *myersBuffer)
     myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
     if (b == NULL) return;
     b->stats.kernelMsTotal = 0.0;
+    b->stats.verifyMsTotal = 0.0;
     b->stats.kernelLaunchesTotal = 0;
     b->stats.batchesTotal = 0;
 }

@@ -614,6 +614,11 @@ __device__ __forceinline__ bool pull_work(uint32_t *heads, uint32_t n, uint32_t 
 template <int L, int W, bool kCut, bool kPersistent>
 constexpr int kWavesPerEU = L != 1 ? 1 : (W >= 28 ? 3 : ((W >= 20 && !kPersistent && !kCut) ? 4 : 1));
 
+// Variants with a threshold-mode kernel: the cut-off (L = 1, W % 4 == 0,
+// W >= 8) or the early stop (multi-lane).
+template <int L, int W>
+constexpr bool kHasCut = (L == 1 && W >= 8 && W % 4 == 0) || (L > 1 && L <= 64);
+
 // Issue priority of a variant's waves (s_setprio).  A wave of a long read
 // runs one candidate's chain of dependent column steps, and the chain of the
 // longest candidate sets the time of its batch; so the arbiter of a shared
@@ -1119,6 +1124,68 @@ __global__ __launch_bounds__(64 * MYERS_LONG_WAVES) void verify_long_kernel(myer
     }
 }
 
+// ------------------------------------------------- one launch over all bins
+// The queue launch: a resident grid of one-wave workgroups takes one chunk
+// at a time from a single counter over the bins in plan order (the host
+// orders them longest reads first, so the grid's tail is short chunks), and
+// runs each chunk with its bin's variant.  No launch boundary separates the
+// bins: a wave that finishes the last chunk of one bin takes the next bin's
+// first.  The register budget of a kernel is that of its largest variant,
+// so the one-lane and the multi-lane variants are two kernels (classes);
+// the multi-lane class stops at 12 words per lane (W = 16 would leave two
+// waves per SIMD for every bin of the launch).
+template <int L, int W>
+constexpr int kQueueClass = L == 1 ? MYERS_QUEUE_ONE_LANE : ((L <= 64 && W <= 12) ? MYERS_QUEUE_MULTI_LANE : -1);
+
+#define MYERS_QUEUE_CASE(L, W)                                                                   \
+    case ((L) << 8) | (W):                                                                       \
+        if constexpr (kQueueClass<(L), (W)> == kClass) {                                         \
+            auto &w = *reinterpret_cast<Workspace<(L), (W)> *>(queue_lds);                       \
+            if (prepared != key) {                                                               \
+                set_wave_prio<(L)>();                                                            \
+                zero_row_init<(L), (W)>(w);                                                      \
+                prepared = key;                                                                  \
+            }                                                                                    \
+            verify_chunks<(L), (W), kCut && kHasCut<(L), (W)>, 1>(a, start, count, chunk, chunk + 1u, 1u, w); \
+        }                                                                                        \
+        break;
+
+template <int kClass, bool kCut>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+void verify_queue_kernel(myers_verify_args_t a, myers_queue_plan_t p) {
+    extern __shared__ uint4 queue_lds[];
+    uint32_t *counter = a.queue_heads + MYERS_QUEUE_HEAD(kClass);
+    uint32_t prepared = 0xFFFFFFFFu;
+    for (;;) {
+        uint32_t got = 0;
+        if (threadIdx.x == 0) got = atomicAdd(counter, 1u);
+        const uint32_t item = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)got, 0));
+        uint32_t j = 0, base = 0, start = 0, count = 0;
+        bool found = false;
+        for (; j < p.n; j++) {
+            const uint32_t v = p.variant[j];
+            const uint32_t c = a.bin_ranges[2u * v + 1u];
+            const uint32_t slots = 64u / (p.key[j] >> 8);
+            const uint32_t chunks = (c + slots - 1u) / slots;
+            if (item - base < chunks) {
+                start = a.bin_ranges[2u * v];
+                count = c;
+                found = true;
+                break;
+            }
+            base += chunks;
+        }
+        if (!found) break;                       // every bin drained: each wave exits here
+        const uint32_t chunk = item - base;
+        const uint32_t key = p.key[j];
+        switch (key) {
+            MYERS_VARIANT_LIST(MYERS_QUEUE_CASE)
+            default:
+                break;
+        }
+    }
+}
+
 // ---------------------------------------------------------------- binning
 
 __global__ __launch_bounds__(64) void bin_reset_kernel(uint32_t *err) {
@@ -1221,6 +1288,7 @@ __global__ __launch_bounds__(64) void bin_base_kernel(uint32_t nv, uint32_t *ran
     if (threadIdx.x == 0) {
         queue_heads[MYERS_LONG_HEAD] = 0;
         queue_heads[MYERS_WIDE_HEAD] = 0;
+        for (uint32_t c = 0; c < (uint32_t)MYERS_QUEUE_CLASSES; c++) queue_heads[MYERS_QUEUE_HEAD(c)] = 0;
     }
 }
 
@@ -1286,8 +1354,6 @@ struct VariantEntry {
     const void *cut_persistent_fn;
     uint32_t block;                  // threads per workgroup
 };
-template <int L, int W>
-constexpr bool kHasCut = (L == 1 && W >= 8 && W % 4 == 0) || (L > 1 && L <= 64);
 template <int L, int W>
 constexpr const void *cut_kernel(bool persistent) {
     if constexpr (kHasCut<L, W>) {
@@ -1368,6 +1434,56 @@ extern "C" int myers_variant_long_ok(uint32_t v, uint32_t *ws_u4) {
 }
 
 extern "C" uint32_t myers_long_wide_lds_bytes(void) { return (uint32_t)sizeof(LongWide); }
+
+namespace {
+#define MYERS_QUEUE_CLASS_OF(L, W) kQueueClass<L, W>,
+const int kQueueClassOf[] = {MYERS_VARIANT_LIST(MYERS_QUEUE_CLASS_OF)};
+const void *queue_kernel(uint32_t cls, bool cut) {
+    if (cls == MYERS_QUEUE_ONE_LANE)
+        return cut ? reinterpret_cast<const void *>(&verify_queue_kernel<MYERS_QUEUE_ONE_LANE, true>)
+                   : reinterpret_cast<const void *>(&verify_queue_kernel<MYERS_QUEUE_ONE_LANE, false>);
+    return cut ? reinterpret_cast<const void *>(&verify_queue_kernel<MYERS_QUEUE_MULTI_LANE, true>)
+               : reinterpret_cast<const void *>(&verify_queue_kernel<MYERS_QUEUE_MULTI_LANE, false>);
+}
+}  // namespace
+
+extern "C" int myers_variant_queue_class(uint32_t v, uint32_t *ws_u4) {
+    constexpr uint32_t kOneWave = sizeof(kQueueClassOf) / sizeof(kQueueClassOf[0]);
+    if (v >= kOneWave || kQueueClassOf[v] < 0) return -1;
+    if (ws_u4) *ws_u4 = kWorkspaceU4[v];
+    return kQueueClassOf[v];
+}
+
+extern "C" int myers_queue_occupancy(uint32_t cls, int cut, uint32_t lds_bytes, int *wg_per_cu) {
+    if (cls >= (uint32_t)MYERS_QUEUE_CLASSES || wg_per_cu == nullptr) return -1;
+    int blocks = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, queue_kernel(cls, cut != 0), 64,
+                                                                (size_t)lds_bytes);
+    if (e != hipSuccess) return (int)e;
+    *wg_per_cu = blocks;
+    return 0;
+}
+
+extern "C" int myers_launch_queue(const myers_verify_args_t *args, const myers_queue_plan_t *plan, uint32_t grid,
+                                  hipStream_t stream) {
+    if (args == nullptr || plan == nullptr || grid == 0 || plan->cls >= (uint32_t)MYERS_QUEUE_CLASSES ||
+        plan->n == 0 || plan->n > (uint32_t)MYERS_QUEUE_MAX || args->bin_ranges == nullptr ||
+        args->queue_heads == nullptr)
+        return -1;
+    uint32_t need = 0;
+    for (uint32_t j = 0; j < plan->n; j++) {
+        uint32_t u4 = 0;
+        const uint32_t v = plan->variant[j];
+        if (myers_variant_queue_class(v, &u4) != (int)plan->cls ||
+            plan->key[j] != ((kVariants[v].lanes << 8) | kVariants[v].words))
+            return -1;
+        need = u4 > need ? u4 : need;
+    }
+    if (plan->ws_u4 < need || plan->ws_u4 * 16u > 64u * 1024u) return -1;
+    void *kargs[] = {const_cast<myers_verify_args_t *>(args), const_cast<myers_queue_plan_t *>(plan)};
+    return (int)hipLaunchKernel(queue_kernel(plan->cls, args->threshold >= 0.0f), dim3(grid), dim3(64), kargs,
+                                (size_t)plan->ws_u4 * 16u, stream);
+}
 
 extern "C" int myers_launch_long(const myers_verify_args_t *args, const myers_long_plan_t *plan, uint32_t grid,
                                  hipStream_t stream) {

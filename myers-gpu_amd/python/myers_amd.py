@@ -30,7 +30,9 @@ WINDOW_EXACT = 1
 ENGINE_AUTO = 0
 ENGINE_PLAIN = 1 << 4
 ENGINE_PERSISTENT = 2 << 4
+ENGINE_BINNED = 3 << 4
 USE_GRAPH = 1 << 6
+NO_LONG_LAUNCH = 1 << 7
 NOALLOC_MAX_CANDIDATES = 1 << 21     # myersVerifyDeviceNoAlloc's static workspace
 
 ERRORS = {
@@ -59,7 +61,8 @@ class Stats(ctypes.Structure):
                 ("kernelMs", ctypes.c_double), ("batchMs", ctypes.c_double),
                 ("cells", ctypes.c_uint64), ("candidates", ctypes.c_uint64),
                 ("kernelMsTotal", ctypes.c_double), ("kernelLaunchesTotal", ctypes.c_uint64),
-                ("batchesTotal", ctypes.c_uint64), ("graphCaptures", ctypes.c_uint64)]
+                ("batchesTotal", ctypes.c_uint64), ("graphCaptures", ctypes.c_uint64),
+                ("verifyMs", ctypes.c_double), ("verifyMsTotal", ctypes.c_double)]
 
 
 class VariantInfo(ctypes.Structure):

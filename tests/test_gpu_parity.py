@@ -91,7 +91,7 @@ def test_mixed_lengths_binned():
     with M.Myers(ref, 2_000_000, avg_query=550, cands_per_query=11, buffer_bytes=256 << 20) as m:
         got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1)
         st = m.stats()
-    assert st.binned == 1 and st.launches >= 4
+    assert st.binned == 1 and st.launches == 1          # one queue launch over the six bins
     np.testing.assert_array_equal(got, oracle_for(ref, 2_000_000, b, 0.1, O.MODE_ALIGNED))
 
 
@@ -238,9 +238,11 @@ def test_repeat_sends_are_independent():
     np.testing.assert_array_equal(first, again)
 
 
-@pytest.mark.parametrize("engine", [M.ENGINE_PLAIN, M.ENGINE_PERSISTENT])
+@pytest.mark.parametrize("engine", [M.ENGINE_PLAIN, M.ENGINE_PERSISTENT, M.ENGINE_BINNED,
+                                    M.ENGINE_PERSISTENT | M.NO_LONG_LAUNCH])
 def test_engines_mixed_lengths(engine):
-    """Plain per-bin launches and persistent queue-draining launches give identical results."""
+    """Plain per-bin launches, the queue launches, per-bin resident grids and
+    the queue launches without the long-read launch give identical results."""
     announce(f"engine {engine}")
     ref, b = synth_case(31, 3_000_000, 900, choices=[64, 100, 200, 333, 600, 1000, 1500, 3000],
                         err_range=(0.01, 0.08), sub_frac=0.7, poisson=9, cap=40, distance=0.1)
@@ -261,7 +263,7 @@ def test_persistent_single_length():
     np.testing.assert_array_equal(got, oracle_for(ref, 1_000_000, b, 0.1, 0))
 
 
-@pytest.mark.parametrize("engine", [M.ENGINE_PLAIN, M.ENGINE_PERSISTENT])
+@pytest.mark.parametrize("engine", [M.ENGINE_PLAIN, M.ENGINE_PERSISTENT, M.ENGINE_BINNED])
 def test_graph_replays_match_eager(engine):
     """A captured pipeline, replayed several times with eager work and
     synchronisations in between, matches the eager result every time."""
@@ -292,7 +294,7 @@ def test_graph_replays_match_eager(engine):
             x.free()
 
 
-@pytest.mark.parametrize("engine", [M.ENGINE_PLAIN, M.ENGINE_PERSISTENT])
+@pytest.mark.parametrize("engine", [M.ENGINE_PLAIN, M.ENGINE_PERSISTENT, M.ENGINE_BINNED])
 def test_multiwave_variants_in_mixed_batches(engine):
     """Long reads on the multi-wave variants (one candidate per workgroup of
     2 or 4 waves) next to one-wave variants in one binned batch."""
@@ -346,7 +348,7 @@ def test_long_read_bins_share_one_launch(mode):
         bins = set(int(table[x]) for x in wq)
         nlong = sum(1 for v in bins if 4 <= variants[v]["lanes"] <= 64)
         assert nlong >= 10
-        assert st.launches == len(bins) - nlong + 1
+        assert st.launches == 2                # the long-read launch and one queue launch for the short bins
         np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b, 0.1, mode))
 
 
@@ -369,7 +371,7 @@ def test_long_read_launch_wide_rounds():
         st = m.stats()
         table = m.bin_table()
         bins = set(int(table[x]) for x in (b["qinfo"][:, 1].astype(np.int64) + 31) // 32)
-        assert st.launches == len(bins) - 2 + 1          # the two long bins share one launch
+        assert len(bins) == 3 and st.launches == 2       # the two long bins share one launch
         np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b, 0.1, O.MODE_ALIGNED))
 
 
@@ -456,3 +458,63 @@ def test_long_read_launch_edge_cases():
         got = m.verify(b3["peq"], b3["qinfo"], b3["cands"], 0.1, flags=M.ENGINE_PERSISTENT)
         assert m.stats().launches == 1
         np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b3, 0.1, O.MODE_ALIGNED))
+
+
+@pytest.mark.parametrize("mode", [M.WINDOW_ALIGNED, M.WINDOW_EXACT])
+def test_queue_launch_config3_mix(mode):
+    """Config-3 mix (six one-lane bins) through the persistent engine: one
+    queue launch over every bin, bit-exact against the oracle."""
+    announce(f"queue launch config-3 mix mode={mode}")
+    ref, b = synth_case(41, 3_000_000, 1500, choices=[100, 200, 400, 600, 800, 1000], err_range=(0.01, 0.05),
+                        sub_frac=0.8, poisson=11, cap=64, distance=0.1)
+    with M.Myers(ref, 3_000_000, avg_query=550, cands_per_query=11, buffer_bytes=256 << 20) as m:
+        got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=mode | M.ENGINE_PERSISTENT)
+        st = m.stats()
+    assert st.binned == 1 and st.launches == 1 and st.variantsUsed == 6
+    np.testing.assert_array_equal(got, oracle_for(ref, 3_000_000, b, 0.1, mode))
+
+
+def test_queue_launch_every_variant():
+    """Every variant of a queue class, forced for every read length up to its
+    capacity, through the queue launch (the long-read launch switched off),
+    and a batch holding a bin of every one-lane and every multi-lane queue
+    variant at once (two queue launches)."""
+    ref_size = 1_000_000
+    ref = M.synth_reference(42, ref_size)
+    with M.Myers(ref, ref_size, avg_query=2000, cands_per_query=6, buffer_bytes=256 << 20) as m:
+        variants = m.variants()
+        default = m.bin_table()
+        queue = [v for v, info in enumerate(variants) if info["lanes"] == 1 or
+                 (info["lanes"] <= 64 and info["words"] <= 12)]
+        assert len(queue) >= 30
+        flags = M.ENGINE_PERSISTENT | M.NO_LONG_LAUNCH
+        for v in queue:
+            info = variants[v]
+            cap = info["maxQuerySize"]
+            announce(f"queue variant {v} lanes={info['lanes']} words={info['words']}")
+            lengths = sorted({max(1, cap - 32 * info["words"] * info["lanes"] + 1), cap // 2 + 1, cap - 17, cap})
+            b = M.synth_batch(M.make_spec(300 + v, 8, choices=[x for x in lengths if x >= 1], sub=0.04,
+                                          ins=0.01, dele=0.01, candidates=6, distance=0.1), ref, ref_size)
+            table = default.copy()
+            w = np.arange(len(table))
+            table[(w >= 1) & (w * 32 <= cap)] = v
+            m.set_bin_table(table)
+            mode = M.WINDOW_EXACT if v % 2 else M.WINDOW_ALIGNED
+            got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=mode | flags)
+            assert m.stats().launches == 1
+            np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b, 0.1, mode), err_msg=f"variant {info}")
+        m.set_bin_table(default)
+        # one bin per queue variant: lengths spread so that each word count maps to its own variant
+        table = default.copy()
+        lens, used = [], []
+        for w in range(1, min(len(table), 24576 // 32 + 1)):
+            if table[w] in queue and table[w] not in used:
+                used.append(int(table[w]))
+                lens.append(32 * w - 5)
+        announce(f"queue launch with {len(used)} bins")
+        b = concat_batches([M.synth_batch(M.make_spec(399 + k, 6 * len(lens[k:k + 8]), choices=lens[k:k + 8],
+                                                      sub=0.04, ins=0.01, dele=0.01, candidates=3, distance=0.1),
+                                          ref, ref_size) for k in range(0, len(lens), 8)])
+        got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=flags)
+        assert m.stats().launches == 2
+        np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b, 0.1, O.MODE_ALIGNED))

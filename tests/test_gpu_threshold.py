@@ -156,3 +156,22 @@ def test_threshold_long_reads_early_stop(mode):
             m.set_threshold(e)
             got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=mode | M.ENGINE_PLAIN)
             np.testing.assert_array_equal(got, expected(ref, size, b, 0.1, mode, e))
+
+
+@pytest.mark.parametrize("mode", [M.WINDOW_ALIGNED, M.WINDOW_EXACT])
+def test_threshold_queue_launches(mode):
+    """Threshold mode through the queue launches of both classes (the cut-off
+    kernels of the one-lane class, the early stop of the multi-lane class)."""
+    size = 3_000_000
+    ref = M.synth_reference(66, size)
+    b = M.synth_batch(M.make_spec(67, 400, choices=[100, 333, 600, 1000, 1500, 3000, 6000, 12000],
+                                  err_range=(0.01, 0.3), sub_frac=0.7, poisson=5, cap=20, distance=0.1),
+                      ref, size)
+    with M.Myers(ref, size, avg_query=3000, cands_per_query=5, buffer_bytes=256 << 20) as m:
+        for e in (0.05, 0.2):
+            announce(f"threshold queue launches mode={mode} e={e}")
+            m.set_threshold(e)
+            got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1,
+                           flags=mode | M.ENGINE_PERSISTENT | M.NO_LONG_LAUNCH)
+            assert m.stats().launches == 2
+            np.testing.assert_array_equal(got, expected(ref, size, b, 0.1, mode, e))

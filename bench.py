@@ -331,7 +331,7 @@ def concat_requests(reqs):
     return np.concatenate(peqs), np.concatenate(qis), np.concatenate(cands)
 
 
-def measure_serving(m, ref, ref_size, loads, seconds, batch_cells=0, engine=0, seed=5):
+def measure_serving(m, ref, ref_size, loads, seconds, batch_cells=0, engine=0, seed=5, resident=True):
     pool = serving_pool(ref, ref_size, seed, 192)
     cells = [int((r[1][r[2]["query"], 1].astype(np.int64) * window_len(r[1][r[2]["query"], 1])).sum()) for r in pool]
     cands = [len(r[2]) for r in pool]
@@ -349,9 +349,13 @@ def measure_serving(m, ref, ref_size, loads, seconds, batch_cells=0, engine=0, s
         if not np.array_equal(whole_res[offs[k]:offs[k + 1]], alone[k]):
             raise SystemExit("bench: results of the pool as one batch differ from the requests verified alone")
     out = {"pool_requests": len(pool), "mean_reads_per_request": float(np.mean([len(r[1]) for r in pool])),
-           "mean_candidates_per_request": float(np.mean(cands)), "buffers": m.num_buffers,
-           "max_batch_cells": batch_cells or "library default (2.5e11)", "engine": engine, "levels": []}
-    with M.Server(m, distance=DISTANCE, flags=engine, max_batch_cells=batch_cells) as sv:
+           "mean_candidates_per_request": float(np.mean(cands)),
+           "server": "resident kernel" if resident else "continuous batching",
+           "buffers": 1 if resident else m.num_buffers,
+           "max_batch_cells": None if resident else (batch_cells or "library default (2.5e11)"),
+           "engine": engine, "levels": []}
+    flags = engine | (M.SERVE_RESIDENT if resident else 0)
+    with M.Server(m, distance=DISTANCE, flags=flags, max_batch_cells=batch_cells) as sv:
         # saturated throughput: every pool request arrives at once
         made = [M.Server.request(*r) for r in pool]
         lat, wall = sv.run_open_loop([x[0] for x in made], np.zeros(len(made), dtype=np.uint64))
@@ -366,7 +370,8 @@ def measure_serving(m, ref, ref_size, loads, seconds, batch_cells=0, engine=0, s
                 raise SystemExit("bench: served results differ from the same requests verified alone")
         rng = np.random.default_rng(seed + 1)
         for load in loads:
-            rate = load * sat_rps
+            # offered load as a fraction of the mix's bulk rate (the whole pool as one batch)
+            rate = load * bulk_cps / float(np.mean(cands))
             n = max(50, int(rate * seconds))
             idx = rng.integers(0, len(pool), size=n)
             arrivals = (np.cumsum(rng.exponential(1.0 / rate, size=n)) * 1e9).astype(np.uint64)
@@ -380,6 +385,7 @@ def measure_serving(m, ref, ref_size, loads, seconds, batch_cells=0, engine=0, s
             out["levels"].append({
                 "load": load, "requests": n, "offered_requests_per_s": rate,
                 "offered_of_bulk": rate * float(np.mean(cands)) / bulk_cps,
+                "offered_of_saturated": rate / sat_rps,
                 "candidates_per_s": nc / (wall / 1e3), "gcups": sum(cells[i] for i in idx) / (wall / 1e3) / 1e9,
                 "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                 "p999_ms": float(np.percentile(lat, 99.9)), "max_ms": float(lat.max()),
@@ -553,12 +559,14 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay the batch pipeline as a captured HIP graph")
     ap.add_argument("--no-compare", action="store_true", help="skip the headline's engine comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget per config (0 = skip)")
-    ap.add_argument("--serve-loads", default="0.1,0.3,0.5,0.7,1.0",
-                    help="serving load levels (fractions of saturation); '' = skip")
+    ap.add_argument("--serve-loads", default="0.1,0.3,0.5,0.7,0.9",
+                    help="serving load levels (fractions of the mix's bulk rate); '' = skip")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--serve-buffers", type=int, default=3)
     ap.add_argument("--serve-batch-cells", type=float, default=0, help="serving batch cap on cells (0 = library default)")
     ap.add_argument("--serve-engine", default="auto", choices=["auto", "plain", "persistent", "binned"])
+    ap.add_argument("--serve-mode", default="resident", choices=["resident", "batches"],
+                    help="serving by one resident kernel, or by continuous batching over the buffers")
     ap.add_argument("--variant", default=None, help="force a kernel variant for every read length, e.g. L2xW16")
     ap.add_argument("--threshold", type=float, default=0.2,
                     help="also time threshold mode at this error rate on config 2 (negative = skip)")
@@ -607,7 +615,8 @@ def main():
     if serve_loads:
         log("serving ...")
         serving = measure_serving(m, ref, cfg["ref_size"], serve_loads, args.serve_seconds,
-                                  int(args.serve_batch_cells), engines[args.serve_engine])
+                                  int(args.serve_batch_cells), engines[args.serve_engine],
+                                  resident=args.serve_mode == "resident")
 
     out = {
         "metric": "candidates/s & GCUPS on 1 MI355X (bit-exact); p99 latency under serving load",

@@ -214,11 +214,22 @@ int      myersLoadTuning(void *myersBuffer, const char *path);
  * concurrently on the device.  Results are copied to each request's output
  * when its batch completes; every candidate's result is the same as when its
  * request is verified alone. */
+/* Server flag: verify with one resident kernel instead of batches.  Each
+ * admitted request is binned by read length on the host into chunks that go
+ * straight into a device-visible work queue, its longest reads first; a
+ * kernel that stays on the device pulls chunks of every admitted request in
+ * arrival order and reports each request as soon as its own chunks are done,
+ * so a request waits neither for a free buffer nor for other requests.
+ * Requests' inputs, results and the queue live in pinned host memory that the
+ * kernel reads and writes directly (arenaBytes); the kernel leaves the device
+ * after ~50 ms without work and is launched again when work arrives. */
+#define MYERS_SERVE_RESIDENT   (1u << 8)
+
 typedef struct {
     uint32_t structSize;
     uint32_t reserved;
     float    distance;          /* window n = (uint32)(m*(1+2*distance)) for every request */
-    uint32_t flags;             /* MYERS_WINDOW_* | MYERS_ENGINE_* */
+    uint32_t flags;             /* MYERS_WINDOW_* | MYERS_ENGINE_* | MYERS_SERVE_RESIDENT */
     uint64_t hbmBudgetBytes;    /* admission budget (0 = 200e9 bytes) */
     uint32_t pollMicros;        /* completion poll period while batches run (0 = 20) */
     uint32_t maxBatchCandidates;/* cap per batch (0 = slot capacity) */
@@ -226,6 +237,8 @@ typedef struct {
                                    that batches stay short and several run at once
                                    (0 = 2.5e11; UINT64_MAX = no cap); a request larger than
                                    the cap runs as a batch of its own */
+    uint64_t arenaBytes;        /* MYERS_SERVE_RESIDENT: pinned host memory for admitted
+                                   requests' inputs and results (0 = 1 GiB) */
 } myersServeConfig_t;
 
 typedef struct {
@@ -249,6 +262,10 @@ typedef struct {
     double   busyMs;            /* sum of batch kernel time */
     double   admitMs;           /* server-thread time forming and sending batches */
     double   completeMs;        /* server-thread time handing results to requests */
+    /* MYERS_SERVE_RESIDENT: the resident kernel's shape (batches counts its launches) */
+    uint32_t residentWorkgroups;   /* one-wave workgroups per launch */
+    uint32_t residentLdsBytes;     /* LDS per workgroup */
+    uint64_t residentItems;        /* work items published */
 } myersServeStats_t;
 
 void myersServeDefaultConfig(myersServeConfig_t *cfg);

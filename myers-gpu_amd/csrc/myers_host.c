@@ -1087,6 +1087,55 @@ static int finish_batch(myers_buffer_t *b, int with_batch_events)
     return MYERS_SUCCESS;
 }
 
+int myers_serve_plan(void *myersBuffer, myers_serve_plan_t *out)
+{
+    myers_buffer_t *b = (myers_buffer_t *)myersBuffer;
+    if (b == NULL || out == NULL) return MYERS_E_INVALID_ARGUMENT;
+    const myers_ctx_t *ctx = b->ctx;
+    memset(out, 0, sizeof(*out));
+    out->device = ctx->device;
+    out->num_cus = ctx->num_cus;
+    out->d_ref = ctx->d_ref;
+    out->ref_words = ctx->ref_words_alloc;
+    out->ref_size = ctx->ref_size;
+    out->threshold = b->threshold;
+    uint32_t var_of[MYERS_SERVE_MAX_WORDS + 1];
+    uint64_t used = 0;
+    const uint32_t maxw = max_supported_words(ctx);
+    for (uint32_t w = 0; w <= maxw && w <= MYERS_SERVE_MAX_WORDS; w++) {
+        uint32_t v = ctx->bin_of_words[w];
+        if (!myers_variant_serve_ok(v, NULL)) {
+            uint32_t best = 0xFFFFFFFFu, best_cost = 0xFFFFFFFFu;
+            for (uint32_t u = 0; u < ctx->num_variants; u++) {
+                if (!myers_variant_serve_ok(u, NULL) || ctx->variants[u].total_words < (w ? w : 1u)) continue;
+                const uint32_t c = variant_cost(&ctx->variants[u]);
+                if (c < best_cost) { best_cost = c; best = u; }
+            }
+            if (best == 0xFFFFFFFFu) break;      /* longer reads: none can run there */
+            v = best;
+        }
+        var_of[w] = v;
+        used |= 1ull << v;
+        out->max_words = w;
+    }
+    /* ranks: the used variants, most words per candidate first */
+    uint32_t order[MYERS_MAX_VARIANTS];
+    const uint32_t n = present_by_cost(ctx, used, order);
+    uint32_t rank_of_var[MYERS_MAX_VARIANTS];
+    for (uint32_t r = 0; r < n; r++) {
+        const myers_variant_desc_t *d = &ctx->variants[order[r]];
+        uint32_t u4 = 0;
+        myers_variant_serve_ok(order[r], &u4);
+        rank_of_var[order[r]] = r;
+        out->rank_key[r] = (d->lanes << 8) | d->words;
+        out->rank_slots[r] = d->slots;
+        if (u4 * 16u > out->lds_bytes) out->lds_bytes = u4 * 16u;
+    }
+    out->num_ranks = n;
+    for (uint32_t w = 0; w <= out->max_words; w++) out->rank_of_words[w] = (uint8_t)rank_of_var[var_of[w]];
+    return MYERS_SUCCESS;
+}
+
 int myers_buffer_ready(void *myersBuffer)
 {
     myers_buffer_t *b = (myers_buffer_t *)myersBuffer;

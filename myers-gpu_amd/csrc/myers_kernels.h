@@ -138,6 +138,93 @@ int myers_queue_occupancy(uint32_t cls, int cut, uint32_t lds_bytes, int *wg_per
 int myers_launch_queue(const myers_verify_args_t *args, const myers_queue_plan_t *plan, uint32_t grid,
                        hipStream_t stream);
 
+/* Resident serving kernel.  A grid of one-wave workgroups stays on the
+ * device and verifies work items published by a host thread: each item is a
+ * chunk (up to 64 / L candidates of one variant) of one request.  Requests,
+ * items, results and completion tickets live in coherent pinned host memory
+ * that the kernel reads and writes directly; the claim counters live in
+ * device memory (dev: MYERS_SERVE_DEV_WORDS words, zeroed once).
+ *   rings:   two item rings, ring 0 for the chunks of long reads (one long
+ *            chain of column steps each), ring 1 for the others.  A wave
+ *            takes a long item only while no other long item runs on its
+ *            SIMD (a per-SIMD flag in simd_busy, keyed by the hardware wave
+ *            id: two chains on one SIMD would halve each other's speed), and
+ *            a short item otherwise.
+ *   publish: the host writes item i of ring r to items[r][i & ring_mask]
+ *            and then the count of published items to host_tail[r *
+ *            MYERS_SERVE_TAIL_STRIDE] (release).
+ *   claim:   a wave takes item `head` of a ring by CAS while head < dtail,
+ *            the device mirror of the ring's host tail; an idle wave
+ *            refreshes the mirrors from host memory, at most once per
+ *            poll_ticks over the whole grid.
+ *   finish:  the wave adds its item's candidates to the request's device
+ *            counter; the wave that completes the request writes the
+ *            request's ticket to flags[slot] after its results are visible.
+ *   exit:    every wave leaves when *host_stop equals the launch's `gen` and
+ *            no item is left, or when no item was claimed for idle_ticks
+ *            (s_memrealtime ticks, 100 MHz); so every launch ends by itself.
+ * Item and request counters are u32 and compared modulo 2^32. */
+typedef struct {
+    uint32_t key;          /* lanes << 8 | words of the chunk's variant */
+    uint32_t req;          /* request slot (index into reqs, below req_mask + 1) */
+    uint32_t first;        /* first slot of the chunk in the request's order array */
+    uint32_t count;        /* candidates in the chunk */
+} myers_serve_item_t;
+
+typedef struct {
+    const uint32_t *peq;       /* qryEntry_t[] as u32[20] */
+    const uint32_t *qinfo;     /* qryInfo_t[] */
+    const void     *cands;     /* candInfo_t[] */
+    void           *results;   /* resEntry_t[], indexed like cands */
+    const uint32_t *order;     /* slot -> candidate, grouped by variant */
+    uint32_t        num_peq_entries;
+    uint32_t        num_queries;
+    uint32_t        num_items_cands;   /* candidates in the request's items */
+    uint32_t        mode;      /* 0 aligned window, 1 exact-start window */
+    float           distance;
+    float           threshold; /* < 0: off */
+    uint64_t        ticket;    /* written to flags[slot] on completion */
+} myers_serve_req_t;
+
+/* dev words: per ring r, head at MYERS_SERVE_HEAD + 64 r and device tail
+ * at MYERS_SERVE_DTAIL + 64 r; then the poll time, stop generation and last
+ * claim time.  simd_busy: MYERS_SERVE_SIMD_KEYS words. */
+enum { MYERS_SERVE_RINGS = 2, MYERS_SERVE_LONG_RING = 0, MYERS_SERVE_SHORT_RING = 1,
+       MYERS_SERVE_HEAD = 0, MYERS_SERVE_DTAIL = 32, MYERS_SERVE_POLL = 128, MYERS_SERVE_STOP = 160,
+       MYERS_SERVE_LAST = 192, MYERS_SERVE_DEV_WORDS = 224, MYERS_SERVE_TAIL_STRIDE = 16,
+       MYERS_SERVE_SIMD_KEYS = 16384, MYERS_SERVE_LONG_LANES = 16 };
+
+typedef struct {
+    const uint32_t *ref;
+    uint64_t        ref_words;
+    uint64_t        ref_size;
+    const myers_serve_item_t *items;   /* host memory: ring r at items + r * (ring_mask + 1) */
+    const myers_serve_req_t  *reqs;    /* request table in host memory */
+    const uint32_t *host_tail;         /* host memory: items published, ring r at r * MYERS_SERVE_TAIL_STRIDE */
+    const uint32_t *host_stop;         /* host memory: generation asked to stop */
+    uint64_t       *flags;             /* host memory: per request slot, ticket when done */
+    uint32_t       *done;              /* device memory: per request slot, candidates done (zeroed) */
+    uint32_t       *dev;               /* device memory: claim counters (zeroed) */
+    uint32_t       *simd_busy;         /* device memory: MYERS_SERVE_SIMD_KEYS flags (zeroed) */
+    uint32_t        ring_mask;         /* items ring size - 1 (power of two) */
+    uint32_t        req_mask;          /* request table size - 1 (power of two) */
+    uint32_t        gen;               /* this launch's generation */
+    uint32_t        idle_ticks;
+    uint32_t        poll_ticks;
+    uint32_t        long_per_simd;     /* long items that may run on one SIMD at once */
+} myers_serve_args_t;
+
+/* 1 if variant v can run in the resident kernel (the one-lane and the
+ * multi-lane queue classes), and its LDS workspace in 16-byte units.  Its
+ * chunks go to the long ring when it has MYERS_SERVE_LONG_LANES lanes or
+ * more per candidate. */
+int myers_variant_serve_ok(uint32_t v, uint32_t *ws_u4);
+int myers_serve_occupancy(int cut, uint32_t lds_bytes, int *wg_per_cu);
+/* grid = one-wave workgroups; lds_bytes >= the largest workspace of the
+ * variants the host will publish. */
+int myers_launch_serve(const myers_serve_args_t *args, int cut, uint32_t grid, uint32_t lds_bytes,
+                       hipStream_t stream);
+
 /* Device-side length binning: a stable partition of the candidates by
  * variant.  bin_of_words[w] (w = ceil(m/32), 0..max_words) gives the variant
  * of a query of w words.  block_counts holds MYERS_BIN_WORDS u32 of

@@ -103,6 +103,31 @@ struct Shape {
     static constexpr int kLdsU4 = kGroups * kGroupU4;
 };
 
+// Loads and stores of a batch's arrays.  kHost: the arrays are host memory
+// rewritten while a resident kernel runs (the serving arena), which the GPU
+// may hold in its L2; system-scope accesses go around that cache, so a load
+// never sees a line cached for an earlier request and a store reaches the
+// host once it completes.  Otherwise plain (cached) accesses.
+template <bool kHost>
+__device__ __forceinline__ uint32_t ld32(const uint32_t *p) {
+    if constexpr (kHost) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    else return *p;
+}
+template <bool kHost>
+__device__ __forceinline__ uint64_t ld64(const uint64_t *p) {
+    if constexpr (kHost) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    else return *p;
+}
+template <bool kHost>
+__device__ __forceinline__ void st_result(uint2 *p, uint2 v) {
+    if constexpr (kHost) {
+        __hip_atomic_store(reinterpret_cast<uint64_t *>(p), ((uint64_t)v.y << 32) | v.x, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+    } else {
+        *p = v;
+    }
+}
+
 // Stage the Peq words of the wave's query slots into LDS, top-justified,
 // with all 64 lanes sharing the work.  Word gw (0 <= gw < L*W) of a slot
 // holds virtual rows [32gw, 32gw+32); virtual row r is query row r - pad,
@@ -111,7 +136,7 @@ struct Shape {
 // j (rows 32j..32j+31) of base c is bitmap[c][j%4] of entry posEntry + j/4;
 // words outside the query read 0.  Lane gl = gw / W of the group reads word
 // k = gw % W at LDS uint4 (k/4)*kGroupU4 + slot*kSlotU4 + c*L + gl, lane k%4.
-template <int L, int W>
+template <int L, int W, bool kHost = false>
 __device__ __forceinline__ void stage_peq(uint4 *lds, const uint2 *qtab, uint32_t nslots,
                                           const uint32_t *__restrict__ peq, uint32_t lane) {
     using S = Shape<L, W>;
@@ -130,8 +155,9 @@ __device__ __forceinline__ void stage_peq(uint4 *lds, const uint2 *qtab, uint32_
         const uint32_t sh = (uint32_t)b0 & 31u;
         const uint32_t *base = peq + (size_t)qi.x * 20u + 4u * c;
         uint32_t wlo = 0, whi = 0;
-        if (lo >= 0 && lo < nq) wlo = base[(uint32_t)(lo >> 2) * 20u + (uint32_t)(lo & 3)];
-        if (lo + 1 >= 0 && lo + 1 < nq) whi = base[(uint32_t)((lo + 1) >> 2) * 20u + (uint32_t)((lo + 1) & 3)];
+        if (lo >= 0 && lo < nq) wlo = ld32<kHost>(base + (uint32_t)(lo >> 2) * 20u + (uint32_t)(lo & 3));
+        if (lo + 1 >= 0 && lo + 1 < nq)
+            whi = ld32<kHost>(base + (uint32_t)((lo + 1) >> 2) * 20u + (uint32_t)((lo + 1) & 3));
         const uint32_t k = gw % (uint32_t)W, gl = gw / (uint32_t)W;
         const uint32_t u4 = (k >> 2) * (uint32_t)S::kGroupU4 + slot * (uint32_t)S::kSlotU4 + c * (uint32_t)L + gl;
         l32[u4 * 4u + (k & 3u)] = (uint32_t)((((uint64_t)whi << 32) | wlo) >> sh);
@@ -315,8 +341,9 @@ __device__ __forceinline__ void zero_row_init(Workspace<L, W> &ws) {
 // Verify chunks chunk_lo, chunk_lo + stride, ... < chunk_hi of the slot range
 // [start, start + count); a chunk is the kSlots candidates of one wave.
 // kCut: threshold mode.  L = 1 (W % 4 == 0): the Ukkonen cut-off; L > 1: a
-// wave stops once its candidates cannot get under the threshold.
-template <int L, int W, bool kCut, int kWaves = 1>
+// wave stops once its candidates cannot get under the threshold.  kHost: the
+// batch's arrays are host memory of the resident server (ld32 / st_result).
+template <int L, int W, bool kCut, int kWaves = 1, bool kHost = false>
 __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint32_t start, uint32_t count,
                                               uint32_t chunk_lo, uint32_t chunk_hi, uint32_t stride,
                                               Workspace<L, W> &ws) {
@@ -343,14 +370,19 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
         bool sane = false;
         if (valid) {
             const uint32_t s = start + slot;
-            cand = a.order ? a.order[s] : s;
-            const CandRec cr = cands[cand];
-            pos = cr.position;
-            query = cr.query;
-            if (cr.query < a.num_queries) {
-                const uint2 qi = qinfo[cr.query];
-                pos_entry = qi.x;
-                m = qi.y;
+            cand = a.order ? ld32<kHost>(a.order + s) : s;
+            if constexpr (kHost) {
+                pos = ld64<true>(&cands[cand].position);
+                query = ld32<true>(&cands[cand].query);
+            } else {
+                const CandRec cr = cands[cand];
+                pos = cr.position;
+                query = cr.query;
+            }
+            if (query < a.num_queries) {
+                const uint64_t qi = ld64<kHost>(reinterpret_cast<const uint64_t *>(qinfo + query));
+                pos_entry = (uint32_t)qi;
+                m = (uint32_t)(qi >> 32);
                 // the query must fit this variant and its Peq entries the batch
                 sane = m <= 32u * S::kTotal &&
                        (uint64_t)pos_entry + ((m + 127u) >> 7) <= (uint64_t)a.num_peq_entries;
@@ -388,7 +420,7 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
             chunk_sync<kWaves>();                        // previous readers of qtab / lds are done
             if (in_pass && head) qtab[my_slot] = make_uint2(pos_entry, m);
             chunk_sync<kWaves>();
-            stage_peq<L, W>(lds, qtab, min(nqslots - base_slot, (uint32_t)S::kQSlots), a.peq, lane);
+            stage_peq<L, W, kHost>(lds, qtab, min(nqslots - base_slot, (uint32_t)S::kQSlots), a.peq, lane);
             chunk_sync<kWaves>();
             const uint4 *lds_lane = lds + my_slot * (uint32_t)S::kSlotU4 + gl;
             const uint32_t ncols = in_pass ? ncols_live : 0u;
@@ -564,7 +596,7 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
                 // threshold mode: score <= (uint32)((float)m * e) (src/checksum-2.c:88,104-105)
                 if (live && a.threshold >= 0.0f && (uint32_t)min_score > threshold_of(m, a.threshold))
                     r = make_uint2(kNoMatch, kNoMatch);
-                results[cand] = r;
+                st_result<kHost>(results + cand, r);
             }
         }
     }
@@ -1186,6 +1218,231 @@ void verify_queue_kernel(myers_verify_args_t a, myers_queue_plan_t p) {
     }
 }
 
+// ------------------------------------------------- resident serving kernel
+// One-wave workgroups that stay resident and verify the chunks a host thread
+// publishes (the protocol is described with myers_serve_args_t).  Loads of
+// host-written descriptors use system-scope atomics: vector loads that no
+// cache keeps stale, which matters for a kernel that outlives many requests.
+// Every atomic is relaxed: on a chip of several XCDs an agent- or
+// system-scope acquire invalidates the XCD's L2 and a release writes it back,
+// and idle waves polling with them would keep flushing the reference text of
+// the working waves.  The ordering the protocol needs comes instead from
+// (a) the host side: items and requests are written before the tail;
+// (b) address dependencies: a wave reads an item only at an index it won,
+//     and a request only at the slot the item names;
+// (c) the request arena, results and tickets live in fine-grained host
+//     memory, which the GPU does not cache, so a store is visible to the host
+//     once it has completed; a wave waits for its result stores to complete
+//     (s_waitcnt, the workgroup-scope release) before it counts its chunk,
+//     so the wave that completes a request writes the ticket after every
+//     result of the request has completed.
+
+__device__ __forceinline__ uint32_t load_sys(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ uint64_t load_sys64(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+// The same value is loaded by every lane: keep it in scalar registers, as a
+// kernel argument would be, so it takes no vector registers in the column loops.
+__device__ __forceinline__ uint32_t uniform_sys(const uint32_t *p) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)load_sys(p));
+}
+__device__ __forceinline__ uint64_t uniform_sys64(const void *p) {
+    const uint64_t v = load_sys64(reinterpret_cast<const uint64_t *>(p));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// A key of the SIMD this wave runs on, from the hardware wave id (SIMD, CU,
+// shader array and engine) and the XCD.  Two waves with the same key share a
+// SIMD; a collision of two SIMDs only costs speed.
+__device__ __forceinline__ uint32_t simd_key() {
+    const uint32_t hw = (uint32_t)__builtin_amdgcn_s_getreg((31 << 11) | 4);     // HW_REG_HW_ID
+    const uint32_t xcc = (uint32_t)__builtin_amdgcn_s_getreg((31 << 11) | 20);   // HW_REG_XCC_ID
+    const uint32_t simd = (hw >> 4) & 3u, cu = (hw >> 8) & 15u, sh = (hw >> 12) & 1u, se = (hw >> 13) & 7u;
+    return (simd | (cu << 2) | (sh << 6) | (se << 7) | ((xcc & 15u) << 10)) & (MYERS_SERVE_SIMD_KEYS - 1u);
+}
+
+// Try to take the next item of ring r: true and *item on success.
+__device__ __forceinline__ bool serve_take(uint32_t *head, uint32_t *dtail, uint32_t &item) {
+    for (;;) {
+        const uint32_t h = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t t = __hip_atomic_load(dtail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((int32_t)(t - h) <= 0) return false;
+        uint32_t expect = h;
+        if (__hip_atomic_compare_exchange_strong(head, &expect, h + 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)) {
+            item = h;
+            return true;
+        }
+    }
+}
+
+// Lane 0: the next item's index in its ring (`ring`), or false when the wave
+// must exit.  A long item is taken only with the SIMD's flag, which the caller clears
+// when the item is done.  The idle time counts from the later of the grid's
+// last claim and this wave's start (the last claim may date from an earlier
+// launch).  An idle wave backs off from ~3 us to ~220 us between polls, so
+// that a grid of idle waves does not flood the memory system with atomics
+// while a few waves run long chains.  The wave's own count of idle polls
+// bounds the wait as well (under ~30 s), whatever the real-time counter does.
+__device__ bool serve_claim(const myers_serve_args_t &s, uint64_t t_start, uint32_t simd, uint32_t &item,
+                            uint32_t &ring) {
+    uint32_t *dstop = s.dev + MYERS_SERVE_STOP;
+    uint64_t *poll = reinterpret_cast<uint64_t *>(s.dev + MYERS_SERVE_POLL);
+    uint64_t *last = reinterpret_cast<uint64_t *>(s.dev + MYERS_SERVE_LAST);
+    uint32_t *busy = s.simd_busy + simd;
+    uint32_t idle = 0, polls = 0;
+    for (;;) {
+        // a long item, if fewer than long_per_simd long items run on this SIMD
+        uint32_t *hl = s.dev + MYERS_SERVE_HEAD, *tl = s.dev + MYERS_SERVE_DTAIL;
+        uint32_t running = __hip_atomic_load(busy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((int32_t)(__hip_atomic_load(tl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
+                      __hip_atomic_load(hl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) > 0 &&
+            running < s.long_per_simd) {
+            if (__hip_atomic_compare_exchange_strong(busy, &running, running + 1u, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                if (serve_take(hl, tl, item)) {
+                    ring = MYERS_SERVE_LONG_RING;
+                    __hip_atomic_store(last, __builtin_amdgcn_s_memrealtime(), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    return true;
+                }
+                __hip_atomic_fetch_sub(busy, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (serve_take(s.dev + MYERS_SERVE_HEAD + 64u, s.dev + MYERS_SERVE_DTAIL + 64u, item)) {
+            ring = MYERS_SERVE_SHORT_RING;
+            __hip_atomic_store(last, __builtin_amdgcn_s_memrealtime(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return true;
+        }
+        // nothing claimable: refresh the mirrors of the host tails (one wave per poll period)
+        const uint64_t now = __builtin_amdgcn_s_memrealtime();
+        uint64_t seen = __hip_atomic_load(poll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        polls++;
+        if ((polls & 255u) == 0u ||
+            (now - seen >= s.poll_ticks &&
+             __hip_atomic_compare_exchange_strong(poll, &seen, now, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT))) {
+            bool fresh = false;
+            for (uint32_t r = 0; r < (uint32_t)MYERS_SERVE_RINGS; r++) {
+                uint32_t *dtail = s.dev + MYERS_SERVE_DTAIL + 64u * r;
+                const uint32_t ht = load_sys(s.host_tail + r * (uint32_t)MYERS_SERVE_TAIL_STRIDE);
+                uint32_t old = __hip_atomic_load(dtail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                while ((int32_t)(ht - old) > 0) {
+                    if (__hip_atomic_compare_exchange_strong(dtail, &old, ht, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT)) {
+                        fresh = true;
+                        break;
+                    }
+                }
+            }
+            if (load_sys(s.host_stop) == s.gen)
+                __hip_atomic_store(dstop, s.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (fresh) {
+                idle = 0;
+                continue;
+            }
+        }
+        if (__hip_atomic_load(dstop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == s.gen) return false;
+        uint64_t l = __hip_atomic_load(last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        l = l > t_start ? l : t_start;
+        if ((now > l && now - l > s.idle_ticks) || polls > (1u << 17)) return false;
+        idle = idle < 64u ? idle + 1u : 64u;
+        for (uint32_t k = 0; k < idle; k++) __builtin_amdgcn_s_sleep(127);
+    }
+}
+
+template <int L, int W>
+constexpr bool kServeVariant = kQueueClass<L, W> >= 0;
+// A long item is a chain of column steps that often has its SIMD to itself:
+// it runs with the interleaved word order (verify_chunks with the wave-local
+// synchronisation of several-wave workgroups, which fits a one-wave
+// workgroup as well), 5-10% faster alone than the phased order.
+template <int L>
+constexpr int kServeWaves = L >= MYERS_SERVE_LONG_LANES ? MYERS_LONG_WAVES : 1;
+
+#define MYERS_SERVE_CASE(L, W)                                                                   \
+    case ((L) << 8) | (W):                                                                       \
+        if constexpr (kServeVariant<(L), (W)>) {                                                 \
+            auto &w = *reinterpret_cast<Workspace<(L), (W)> *>(serve_lds);                       \
+            if (prepared != key) {                                                               \
+                set_wave_prio<(L)>();                                                            \
+                zero_row_init<(L), (W)>(w);                                                      \
+                prepared = key;                                                                  \
+            }                                                                                    \
+            verify_chunks<(L), (W), kCut && kHasCut<(L), (W)>, kServeWaves<(L)>, true>(a, first, count, 0u, 1u, 1u, w); \
+        }                                                                                        \
+        break;
+
+template <bool kCut>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+void verify_serve_kernel(myers_serve_args_t s) {
+    extern __shared__ uint4 serve_lds[];
+    uint32_t prepared = 0xFFFFFFFFu;
+    const uint64_t t_start = __builtin_amdgcn_s_memrealtime();
+    const uint32_t simd = simd_key();
+    for (;;) {
+        uint32_t got = 0, got_ring = 2u;           // ring 2: exit
+        if (threadIdx.x == 0 && !serve_claim(s, t_start, simd, got, got_ring)) got_ring = 2u;
+        const uint32_t ring = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)got_ring, 0));
+        if (ring > 1u) break;
+        const uint32_t item = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)got, 0));
+        const myers_serve_item_t *it = s.items + ring * (s.ring_mask + 1u) + (item & s.ring_mask);
+        const uint32_t key = uniform_sys(&it->key);
+        const uint32_t slot = uniform_sys(&it->req) & s.req_mask;
+        const uint32_t first = uniform_sys(&it->first);
+        const uint32_t count = min(uniform_sys(&it->count), 64u);
+        const myers_serve_req_t *rq = s.reqs + slot;
+        myers_verify_args_t a;
+        a.ref = s.ref;
+        a.ref_words = s.ref_words;
+        a.ref_size = s.ref_size;
+        a.peq = reinterpret_cast<const uint32_t *>(uniform_sys64(&rq->peq));
+        a.qinfo = reinterpret_cast<const uint32_t *>(uniform_sys64(&rq->qinfo));
+        a.cands = reinterpret_cast<const void *>(uniform_sys64(&rq->cands));
+        a.results = reinterpret_cast<void *>(uniform_sys64(&rq->results));
+        a.order = reinterpret_cast<const uint32_t *>(uniform_sys64(&rq->order));
+        a.num_peq_entries = uniform_sys(&rq->num_peq_entries);
+        a.num_queries = uniform_sys(&rq->num_queries);
+        a.mode = uniform_sys(&rq->mode);
+        a.distance = __uint_as_float(uniform_sys(reinterpret_cast<const uint32_t *>(&rq->distance)));
+        a.threshold = __uint_as_float(uniform_sys(reinterpret_cast<const uint32_t *>(&rq->threshold)));
+        a.bin_ranges = nullptr;
+        a.start = first;
+        a.count = count;
+        a.variant = 0;
+        a.chunk_factor = 1;
+        a.queue_heads = nullptr;
+        const uint32_t total = uniform_sys(&rq->num_items_cands);
+        const uint64_t ticket = uniform_sys64(&rq->ticket);
+        switch (key) {
+            MYERS_VARIANT_LIST(MYERS_SERVE_CASE)
+            default:
+                break;
+        }
+        // this wave's result stores complete before its chunk is counted
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t before = __hip_atomic_fetch_add(s.done + slot, count, __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+            if (before + count == total) {
+                // the request's last item: reset its counter for the slot's next
+                // request, then publish the ticket (every result has completed)
+                __hip_atomic_store(s.done + slot, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __builtin_amdgcn_s_waitcnt(0);
+                __hip_atomic_store(s.flags + slot, ticket, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            // the SIMD can take another long item
+            if (ring == MYERS_SERVE_LONG_RING)
+                __hip_atomic_fetch_sub(s.simd_busy + simd, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- binning
 
 __global__ __launch_bounds__(64) void bin_reset_kernel(uint32_t *err) {
@@ -1547,6 +1804,33 @@ extern "C" int myers_scratch_init(int device, uint32_t **ptr) {
     }
     *ptr = g_scratch_ptr[device];
     return 0;
+}
+
+extern "C" int myers_variant_serve_ok(uint32_t v, uint32_t *ws_u4) {
+    return myers_variant_queue_class(v, ws_u4) >= 0 ? 1 : 0;
+}
+
+extern "C" int myers_serve_occupancy(int cut, uint32_t lds_bytes, int *wg_per_cu) {
+    if (wg_per_cu == nullptr) return -1;
+    const void *fn = cut ? reinterpret_cast<const void *>(&verify_serve_kernel<true>)
+                         : reinterpret_cast<const void *>(&verify_serve_kernel<false>);
+    int blocks = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, 64, (size_t)lds_bytes);
+    if (e != hipSuccess) return (int)e;
+    *wg_per_cu = blocks;
+    return 0;
+}
+
+extern "C" int myers_launch_serve(const myers_serve_args_t *args, int cut, uint32_t grid, uint32_t lds_bytes,
+                                  hipStream_t stream) {
+    if (args == nullptr || grid == 0 || lds_bytes > 64u * 1024u || args->dev == nullptr || args->done == nullptr ||
+        args->simd_busy == nullptr ||
+        ((args->ring_mask + 1u) & args->ring_mask) != 0u || ((args->req_mask + 1u) & args->req_mask) != 0u)
+        return -1;
+    const void *fn = cut ? reinterpret_cast<const void *>(&verify_serve_kernel<true>)
+                         : reinterpret_cast<const void *>(&verify_serve_kernel<false>);
+    void *kargs[] = {const_cast<myers_serve_args_t *>(args)};
+    return (int)hipLaunchKernel(fn, dim3(grid), dim3(64), kargs, (size_t)lds_bytes, stream);
 }
 
 extern "C" int myers_launch_binning(const uint32_t *qinfo, uint32_t num_queries, const void *cands,

@@ -17,6 +17,17 @@
  *
  * A request's candidates go through exactly the kernels a lone request
  * would, so its results do not depend on what it was batched with.
+ *
+ * Resident mode (MYERS_SERVE_RESIDENT) replaces the batches by one kernel
+ * that stays on the device (verify_serve_kernel): the server thread copies
+ * each admitted request into a pinned arena, bins its candidates by read
+ * length on the host (longest reads first) into chunk items, appends them to
+ * a work queue in pinned memory and publishes the new tail; the kernel's
+ * waves pull items in queue order, and the wave that finishes a request's
+ * last item writes the request's ticket, which the server thread polls.
+ * Admission: a free request slot, room in the item ring and the arena, and
+ * the HBM budget.  The same per-chunk verification runs as in a batch, so a
+ * request's results are again independent of what else is in flight.
  */
 #define _GNU_SOURCE
 #include <errno.h>
@@ -26,9 +37,12 @@
 #include <string.h>
 #include <time.h>
 
+#include <hip/hip_runtime_api.h>
+
 #include "myers-interface.h"
 #include "myers-amd.h"
 #include "myers_internal.h"
+#include "myers_kernels.h"
 
 enum { REQ_QUEUED = 0, REQ_ADMITTED = 1, REQ_DONE = 2 };
 
@@ -43,7 +57,47 @@ typedef struct req {
     int            state;
     int            error;
     struct req    *next;
+    /* resident mode */
+    uint64_t       arena_start, arena_end;    /* monotone arena offsets held */
+    uint64_t       admit_idx;                 /* admission order */
+    uint32_t       first_item[MYERS_SERVE_RINGS], nitems[MYERS_SERVE_RINGS];
+    uint32_t       items;                     /* in both rings */
+    resEntry_t    *res;                       /* results in the arena */
+    int            too_long;
 } req_t;
+
+/* Resident mode state (MYERS_SERVE_RESIDENT). */
+#define RES_RING_ITEMS  (1u << 20)       /* work items per ring (16 MB) */
+#define RES_REQ_SLOTS   4096u            /* requests in flight at most */
+#define RES_IDLE_TICKS  5000000u         /* 50 ms of the 100 MHz real-time counter */
+#define RES_POLL_TICKS  200u             /* 2 us between reads of the host tail */
+#define RES_DEFAULT_ARENA (1ull << 30)
+#define RES_CTL_TAIL    0u               /* ring r's tail at RES_CTL_TAIL + r * MYERS_SERVE_TAIL_STRIDE */
+#define RES_CTL_STOP    32u
+
+typedef struct {
+    myers_serve_plan_t  plan;
+    int                 cut;
+    uint32_t            grid;
+    hipStream_t         stream;
+    hipEvent_t          ev_done;          /* recorded after each launch */
+    int                 running;          /* launched and its end not yet seen */
+    uint32_t            gen;              /* generation of the current launch */
+    myers_serve_item_t *items;            /* pinned, coherent */
+    myers_serve_req_t  *reqs;
+    uint64_t           *flags;
+    uint32_t           *ctl;              /* ring tails (published items), stop generation */
+    uint8_t            *arena;
+    void               *d_items, *d_reqs, *d_flags, *d_ctl, *d_arena;   /* device addresses of the above */
+    uint64_t            arena_bytes, arena_head, arena_tail;
+    uint32_t           *d_dev, *d_done;   /* device memory: claim counters, per-slot counts */
+    uint32_t           *d_simd;           /* device memory: per-SIMD long-item flags */
+    uint32_t            published[MYERS_SERVE_RINGS];
+    uint32_t            long_per_simd;
+    uint32_t            active;           /* admitted requests with items, not yet done */
+    uint64_t            admitted, oldest;
+    req_t             **inflight;         /* RES_REQ_SLOTS ring by admission index */
+} resident_t;
 
 typedef struct {
     void     *buf;
@@ -70,6 +124,7 @@ typedef struct {
     int                stopping;
     int                thread_started;
     myersServeStats_t  stats;
+    resident_t        *res;               /* MYERS_SERVE_RESIDENT */
 } server_t;
 
 static uint64_t now_ns(void)
@@ -239,6 +294,340 @@ static int launch_batch(server_t *sv, slot_t *s)
     return 1;
 }
 
+/* ------------------------------------------------------- resident mode */
+
+static void finish_request(server_t *sv, req_t *q, int error, uint64_t t)
+{
+    q->error = error;
+    q->t_done_ns = t;
+    q->state = REQ_DONE;
+    sv->stats.requests++;
+    sv->stats.candidates += q->r.numCandidates;
+}
+
+static void res_free(resident_t *rs)
+{
+    if (rs == NULL) return;
+    if (rs->running && rs->ev_done) hipEventSynchronize(rs->ev_done);
+    if (rs->ev_done) hipEventDestroy(rs->ev_done);
+    if (rs->stream) hipStreamDestroy(rs->stream);
+    if (rs->items) hipHostFree(rs->items);
+    if (rs->reqs) hipHostFree(rs->reqs);
+    if (rs->flags) hipHostFree(rs->flags);
+    if (rs->ctl) hipHostFree(rs->ctl);
+    if (rs->arena) hipHostFree(rs->arena);
+    if (rs->d_dev) hipFree(rs->d_dev);
+    if (rs->d_done) hipFree(rs->d_done);
+    if (rs->d_simd) hipFree(rs->d_simd);
+    free(rs->inflight);
+    free(rs);
+}
+
+#define RES_TRY(call) do { if ((call) != hipSuccess) { res_free(rs); return MYERS_E_HIP; } } while (0)
+
+/* Pinned, coherent host memory the kernel reads and writes in place. */
+static hipError_t res_host_alloc(void **host, void **dev, size_t bytes)
+{
+    hipError_t e = hipHostMalloc(host, bytes, hipHostMallocCoherent | hipHostMallocMapped);
+    if (e != hipSuccess) return e;
+    memset(*host, 0, bytes);
+    return hipHostGetDevicePointer(dev, *host, 0);
+}
+
+static int res_init(server_t *sv, void *buf)
+{
+    resident_t *rs = (resident_t *)calloc(1, sizeof(resident_t));
+    if (rs == NULL) return MYERS_E_ALLOCATE_MEM;
+    int rc = myers_serve_plan(buf, &rs->plan);
+    if (rc) { free(rs); return rc; }
+    if (rs->plan.num_ranks == 0) { free(rs); return MYERS_E_INVALID_ARGUMENT; }
+    rs->inflight = (req_t **)calloc(RES_REQ_SLOTS, sizeof(req_t *));
+    if (rs->inflight == NULL) { free(rs); return MYERS_E_ALLOCATE_MEM; }
+    rs->cut = rs->plan.threshold >= 0.0f;
+    /* One long item per SIMD: the grid holds two waves per SIMD (registers),
+     * and a second long chain would take the wave that serves short items,
+     * which then starve under load (measured: the served rate halves at 0.7
+     * of the bulk rate). */
+    rs->long_per_simd = 1u;
+    rs->arena_bytes = sv->cfg.arenaBytes ? sv->cfg.arenaBytes : RES_DEFAULT_ARENA;
+    RES_TRY(hipSetDevice(rs->plan.device));
+    RES_TRY(hipStreamCreateWithFlags(&rs->stream, hipStreamNonBlocking));
+    RES_TRY(hipEventCreateWithFlags(&rs->ev_done, hipEventDisableTiming));
+    RES_TRY(res_host_alloc((void **)&rs->items, &rs->d_items,
+                           sizeof(myers_serve_item_t) * RES_RING_ITEMS * MYERS_SERVE_RINGS));
+    RES_TRY(res_host_alloc((void **)&rs->reqs, &rs->d_reqs, sizeof(myers_serve_req_t) * RES_REQ_SLOTS));
+    RES_TRY(res_host_alloc((void **)&rs->flags, &rs->d_flags, sizeof(uint64_t) * RES_REQ_SLOTS));
+    RES_TRY(res_host_alloc((void **)&rs->ctl, &rs->d_ctl, 256));
+    RES_TRY(res_host_alloc((void **)&rs->arena, &rs->d_arena, rs->arena_bytes));
+    RES_TRY(hipMalloc((void **)&rs->d_dev, sizeof(uint32_t) * MYERS_SERVE_DEV_WORDS));
+    RES_TRY(hipMalloc((void **)&rs->d_done, sizeof(uint32_t) * RES_REQ_SLOTS));
+    RES_TRY(hipMalloc((void **)&rs->d_simd, sizeof(uint32_t) * MYERS_SERVE_SIMD_KEYS));
+    RES_TRY(hipMemset(rs->d_simd, 0, sizeof(uint32_t) * MYERS_SERVE_SIMD_KEYS));
+    RES_TRY(hipMemset(rs->d_dev, 0, sizeof(uint32_t) * MYERS_SERVE_DEV_WORDS));
+    RES_TRY(hipMemset(rs->d_done, 0, sizeof(uint32_t) * RES_REQ_SLOTS));
+    RES_TRY(hipDeviceSynchronize());
+    int occ = 0;
+    if (myers_serve_occupancy(rs->cut, rs->plan.lds_bytes, &occ) != 0 || occ <= 0) occ = 1;
+    rs->grid = (uint32_t)rs->plan.num_cus * (uint32_t)occ;
+    sv->stats.residentWorkgroups = rs->grid;
+    sv->stats.residentLdsBytes = rs->plan.lds_bytes;
+    sv->res = rs;
+    return MYERS_SUCCESS;
+}
+
+#define RES_ALIGN(x) (((x) + 255u) & ~(uint64_t)255u)
+
+/* Admit the queue's head request if it fits now: 1 admitted (or failed for
+ * good), 0 it must wait for completions.  Caller holds the lock. */
+static int res_admit(server_t *sv, req_t *q)
+{
+    resident_t *rs = sv->res;
+    const myers_serve_plan_t *pl = &rs->plan;
+    const myersRequest_t *r = &q->r;
+    const uint32_t nr = pl->num_ranks;
+    uint32_t counts[64] = {0}, pos[64];
+    uint32_t too_long = 0, nitems = 0, nring[MYERS_SERVE_RINGS] = {0, 0};
+    for (uint32_t i = 0; i < r->numCandidates; i++) {
+        const uint32_t w = (r->qinfo[r->cands[i].query].size + 31u) >> 5;
+        if (w > pl->max_words) too_long++;
+        else counts[pl->rank_of_words[w]]++;
+    }
+    for (uint32_t k = 0; k < nr; k++) {
+        const uint32_t n = (counts[k] + pl->rank_slots[k] - 1u) / pl->rank_slots[k];
+        nring[(pl->rank_key[k] >> 8) >= (uint32_t)MYERS_SERVE_LONG_LANES ? 0 : 1] += n;
+        nitems += n;
+    }
+    const uint64_t o_peq = 0;
+    const uint64_t o_qi = RES_ALIGN(o_peq + (uint64_t)r->numPEQEntries * sizeof(qryEntry_t));
+    const uint64_t o_ca = RES_ALIGN(o_qi + (uint64_t)r->numQueries * sizeof(qryInfo_t));
+    const uint64_t o_or = RES_ALIGN(o_ca + (uint64_t)r->numCandidates * sizeof(candInfo_t));
+    const uint64_t o_re = RES_ALIGN(o_or + (uint64_t)r->numCandidates * sizeof(uint32_t));
+    const uint64_t len = RES_ALIGN(o_re + (uint64_t)r->numCandidates * sizeof(resEntry_t));
+    const uint64_t t = now_ns();
+    if (len > rs->arena_bytes || nring[0] > RES_RING_ITEMS || nring[1] > RES_RING_ITEMS) {
+        q->t_admit_ns = t;
+        finish_request(sv, q, MYERS_E_CAPACITY, t);
+        return 1;
+    }
+    const int busy = rs->oldest != rs->admitted;
+    if (busy) {
+        if (rs->admitted - rs->oldest >= RES_REQ_SLOTS) return 0;
+        const req_t *old = rs->inflight[rs->oldest % RES_REQ_SLOTS];
+        for (uint32_t g = 0; g < (uint32_t)MYERS_SERVE_RINGS; g++)
+            if ((uint64_t)(uint32_t)(rs->published[g] - old->first_item[g]) + nring[g] > RES_RING_ITEMS) return 0;
+        if (sv->inflight_bytes + q->bytes > sv->cfg.hbmBudgetBytes) return 0;
+    }
+    /* arena: contiguous bytes, skipping the end of the ring when they do not fit there */
+    uint64_t start = rs->arena_head;
+    const uint64_t at = start % rs->arena_bytes;
+    if (at + len > rs->arena_bytes) start += rs->arena_bytes - at;
+    if (start + len - rs->arena_tail > rs->arena_bytes) return 0;
+    rs->arena_head = start + len;
+    const uint64_t base = start % rs->arena_bytes;
+    uint8_t *h = rs->arena + base;
+    const uintptr_t d = (uintptr_t)rs->d_arena + base;
+    memcpy(h + o_peq, r->peq, (size_t)r->numPEQEntries * sizeof(qryEntry_t));
+    memcpy(h + o_qi, r->qinfo, (size_t)r->numQueries * sizeof(qryInfo_t));
+    memcpy(h + o_ca, r->cands, (size_t)r->numCandidates * sizeof(candInfo_t));
+    uint32_t *order = (uint32_t *)(h + o_or);
+    resEntry_t *res = (resEntry_t *)(h + o_re);
+    /* stable counting sort by rank (longest reads first); too-long reads get the sentinel */
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < nr; k++) { pos[k] = acc; acc += counts[k]; }
+    for (uint32_t i = 0; i < r->numCandidates; i++) {
+        const uint32_t w = (r->qinfo[r->cands[i].query].size + 31u) >> 5;
+        if (w > pl->max_words) {
+            res[i].column = 0;
+            res[i].score = 0xFFFFFFFFu;
+            continue;
+        }
+        order[pos[pl->rank_of_words[w]]++] = i;
+    }
+    const uint32_t slot = (uint32_t)(rs->admitted % RES_REQ_SLOTS);
+    myers_serve_req_t *rq = &rs->reqs[slot];
+    rq->peq = (const uint32_t *)(d + o_peq);
+    rq->qinfo = (const uint32_t *)(d + o_qi);
+    rq->cands = (const void *)(d + o_ca);
+    rq->results = (void *)(d + o_re);
+    rq->order = (const uint32_t *)(d + o_or);
+    rq->num_peq_entries = r->numPEQEntries;
+    rq->num_queries = r->numQueries;
+    rq->num_items_cands = r->numCandidates - too_long;
+    rq->mode = sv->cfg.flags & 1u;
+    rq->distance = sv->cfg.distance;
+    rq->threshold = pl->threshold;
+    rq->ticket = q->ticket + 1u;       /* flags start at 0 */
+    uint32_t item[MYERS_SERVE_RINGS] = {rs->published[0], rs->published[1]};
+    acc = 0;
+    for (uint32_t k = 0; k < nr; k++) {
+        const uint32_t g = (pl->rank_key[k] >> 8) >= (uint32_t)MYERS_SERVE_LONG_LANES ? 0u : 1u;
+        for (uint32_t c = 0; c < counts[k]; c += pl->rank_slots[k]) {
+            myers_serve_item_t *it = &rs->items[(size_t)g * RES_RING_ITEMS + (item[g] & (RES_RING_ITEMS - 1u))];
+            it->key = pl->rank_key[k];
+            it->req = slot;
+            it->first = acc + c;
+            it->count = counts[k] - c < pl->rank_slots[k] ? counts[k] - c : pl->rank_slots[k];
+            item[g]++;
+        }
+        acc += counts[k];
+    }
+    q->arena_start = start;
+    q->arena_end = start + len;
+    q->admit_idx = rs->admitted;
+    for (uint32_t g = 0; g < (uint32_t)MYERS_SERVE_RINGS; g++) {
+        q->first_item[g] = rs->published[g];
+        q->nitems[g] = nring[g];
+    }
+    q->items = nitems;
+    q->res = res;
+    q->too_long = too_long != 0;
+    q->t_admit_ns = t;
+    q->state = REQ_ADMITTED;
+    rs->inflight[slot] = q;
+    rs->admitted++;
+    sv->inflight_bytes += q->bytes;
+    if (nitems == 0) return 1;                  /* completed by the next poll */
+    rs->active++;
+    sv->stats.residentItems += nitems;
+    /* publish: the items and the request are written before the tails */
+    for (uint32_t g = 0; g < (uint32_t)MYERS_SERVE_RINGS; g++) {
+        if (item[g] == rs->published[g]) continue;
+        rs->published[g] = item[g];
+        __atomic_store_n(&rs->ctl[RES_CTL_TAIL + g * MYERS_SERVE_TAIL_STRIDE], item[g], __ATOMIC_RELEASE);
+    }
+    return 1;
+}
+
+/* Hand back finished requests and release their arena and slots, oldest
+ * first.  Caller holds the lock. */
+static int res_complete(server_t *sv)
+{
+    resident_t *rs = sv->res;
+    int progressed = 0;
+    const uint64_t t0 = now_ns();
+    for (uint64_t i = rs->oldest; i != rs->admitted; i++) {
+        req_t *q = rs->inflight[i % RES_REQ_SLOTS];
+        if (q == NULL || q->state != REQ_ADMITTED) continue;
+        const uint32_t slot = (uint32_t)(i % RES_REQ_SLOTS);
+        if (q->items && __atomic_load_n(&rs->flags[slot], __ATOMIC_ACQUIRE) != q->ticket + 1u) continue;
+        if (q->items) rs->active--;
+        memcpy(q->r.results, q->res, (size_t)q->r.numCandidates * sizeof(resEntry_t));
+        sv->inflight_bytes -= q->bytes;
+        finish_request(sv, q, q->too_long ? MYERS_E_QUERY_TOO_LONG : MYERS_SUCCESS, now_ns());
+        progressed = 1;
+    }
+    while (rs->oldest != rs->admitted) {
+        req_t *q = rs->inflight[rs->oldest % RES_REQ_SLOTS];
+        if (q != NULL && q->state != REQ_DONE) break;
+        rs->inflight[rs->oldest % RES_REQ_SLOTS] = NULL;
+        rs->oldest++;
+        rs->arena_tail = rs->oldest == rs->admitted ? rs->arena_head
+                                                    : rs->inflight[rs->oldest % RES_REQ_SLOTS]->arena_start;
+    }
+    if (progressed) {
+        sv->stats.completeMs += (double)(now_ns() - t0) / 1e6;
+        pthread_cond_broadcast(&sv->cv_done);
+    }
+    return progressed;
+}
+
+/* Launch the kernel when requests wait on it and no launch is running. */
+static int res_ensure_running(server_t *sv)
+{
+    resident_t *rs = sv->res;
+    if (rs->active == 0) return MYERS_SUCCESS;
+    if (rs->running) {
+        hipError_t e = hipEventQuery(rs->ev_done);
+        if (e == hipErrorNotReady) return MYERS_SUCCESS;
+        if (e != hipSuccess) return MYERS_E_HIP;
+        rs->running = 0;                        /* it went idle and left: start another */
+    }
+    rs->gen = rs->gen + 1u ? rs->gen + 1u : 1u;
+    myers_serve_args_t a;
+    memset(&a, 0, sizeof(a));
+    a.ref = rs->plan.d_ref;
+    a.ref_words = rs->plan.ref_words;
+    a.ref_size = rs->plan.ref_size;
+    a.items = (const myers_serve_item_t *)rs->d_items;
+    a.reqs = (const myers_serve_req_t *)rs->d_reqs;
+    a.host_tail = (const uint32_t *)rs->d_ctl + RES_CTL_TAIL;
+    a.host_stop = (const uint32_t *)rs->d_ctl + RES_CTL_STOP;
+    a.flags = (uint64_t *)rs->d_flags;
+    a.done = rs->d_done;
+    a.dev = rs->d_dev;
+    a.simd_busy = rs->d_simd;
+    a.ring_mask = RES_RING_ITEMS - 1u;
+    a.req_mask = RES_REQ_SLOTS - 1u;
+    a.gen = rs->gen;
+    a.idle_ticks = RES_IDLE_TICKS;
+    a.long_per_simd = rs->long_per_simd;
+    a.poll_ticks = RES_POLL_TICKS;
+    if (myers_launch_serve(&a, rs->cut, rs->grid, rs->plan.lds_bytes, rs->stream) != 0) return MYERS_E_HIP;
+    if (hipEventRecord(rs->ev_done, rs->stream) != hipSuccess) return MYERS_E_HIP;
+    rs->running = 1;
+    sv->stats.batches++;
+    return MYERS_SUCCESS;
+}
+
+/* A failed launch fails every request in flight. */
+static void res_fail_inflight(server_t *sv, int rc)
+{
+    resident_t *rs = sv->res;
+    const uint64_t t = now_ns();
+    for (uint64_t i = rs->oldest; i != rs->admitted; i++) {
+        req_t *q = rs->inflight[i % RES_REQ_SLOTS];
+        if (q == NULL || q->state != REQ_ADMITTED) continue;
+        sv->inflight_bytes -= q->bytes;
+        finish_request(sv, q, rc, t);
+    }
+    rs->active = 0;
+    pthread_cond_broadcast(&sv->cv_done);
+}
+
+static void *resident_main(void *arg)
+{
+    server_t *sv = (server_t *)arg;
+    resident_t *rs = sv->res;
+    hipSetDevice(rs->plan.device);
+    pthread_mutex_lock(&sv->mu);
+    for (;;) {
+        res_complete(sv);
+        const uint64_t t0 = now_ns();
+        int admitted = 0;
+        while (sv->head) {
+            req_t *q = sv->head;
+            if (!res_admit(sv, q)) break;
+            sv->head = q->next;
+            if (!sv->head) sv->tail = NULL;
+            admitted = 1;
+        }
+        if (admitted) {
+            sv->stats.admitMs += (double)(now_ns() - t0) / 1e6;
+            pthread_cond_broadcast(&sv->cv_done);     /* requests failed at admission */
+        }
+        int rc = res_ensure_running(sv);
+        if (rc != MYERS_SUCCESS) res_fail_inflight(sv, rc);
+        if (rs->oldest == rs->admitted) {
+            if (sv->head) continue;
+            if (sv->stopping) break;
+            pthread_cond_wait(&sv->cv_work, &sv->mu);
+            continue;
+        }
+        pthread_mutex_unlock(&sv->mu);
+        struct timespec ts = {0, (long)sv->cfg.pollMicros * 1000L};
+        nanosleep(&ts, NULL);
+        pthread_mutex_lock(&sv->mu);
+    }
+    /* every request is done: ask the running launch to leave, and wait for it */
+    __atomic_store_n(&rs->ctl[RES_CTL_STOP], rs->gen, __ATOMIC_RELEASE);
+    pthread_mutex_unlock(&sv->mu);
+    if (rs->running) {
+        hipEventSynchronize(rs->ev_done);
+        rs->running = 0;
+    }
+    return NULL;
+}
+
 static void *server_main(void *arg)
 {
     server_t *sv = (server_t *)arg;
@@ -288,6 +677,14 @@ int myersServeStart(void **server, void **myersBuffers, uint32_t numBuffers, con
         return MYERS_E_ALLOCATE_MEM;
     }
     sv->nslots = numBuffers;
+    if (sv->cfg.flags & MYERS_SERVE_RESIDENT) {
+        int rc = res_init(sv, myersBuffers[0]);
+        if (rc != MYERS_SUCCESS) {
+            free(sv->slots);
+            free(sv);
+            return rc;
+        }
+    }
     for (uint32_t i = 0; i < numBuffers; i++) {
         sv->slots[i].buf = myersBuffers[i];
         sv->slots[i].max_entries = getMaxPEQEntries(myersBuffers[i]);
@@ -297,7 +694,8 @@ int myersServeStart(void **server, void **myersBuffers, uint32_t numBuffers, con
     pthread_mutex_init(&sv->mu, NULL);
     pthread_cond_init(&sv->cv_work, NULL);
     pthread_cond_init(&sv->cv_done, NULL);
-    if (pthread_create(&sv->thread, NULL, server_main, sv) != 0) {
+    if (pthread_create(&sv->thread, NULL, sv->res ? resident_main : server_main, sv) != 0) {
+        res_free(sv->res);
         free(sv->slots);
         free(sv);
         return MYERS_E_ALLOCATE_MEM;
@@ -398,6 +796,7 @@ int myersServeStop(void *server)
     pthread_cond_signal(&sv->cv_work);
     pthread_mutex_unlock(&sv->mu);
     if (sv->thread_started) pthread_join(sv->thread, NULL);
+    res_free(sv->res);
     for (uint64_t i = 0; i < sv->next_ticket; i++) free(sv->table[i]);
     free(sv->table);
     for (uint32_t i = 0; i < sv->nslots; i++) {

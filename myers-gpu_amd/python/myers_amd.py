@@ -33,6 +33,7 @@ ENGINE_PERSISTENT = 2 << 4
 ENGINE_BINNED = 3 << 4
 USE_GRAPH = 1 << 6
 NO_LONG_LAUNCH = 1 << 7
+SERVE_RESIDENT = 1 << 8              # myersServeConfig_t.flags: one resident kernel instead of batches
 NOALLOC_MAX_CANDIDATES = 1 << 21     # myersVerifyDeviceNoAlloc's static workspace
 
 ERRORS = {
@@ -74,7 +75,8 @@ class VariantInfo(ctypes.Structure):
 class ServeConfig(ctypes.Structure):
     _fields_ = [("structSize", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("distance", ctypes.c_float),
                 ("flags", ctypes.c_uint32), ("hbmBudgetBytes", ctypes.c_uint64), ("pollMicros", ctypes.c_uint32),
-                ("maxBatchCandidates", ctypes.c_uint32), ("maxBatchCells", ctypes.c_uint64)]
+                ("maxBatchCandidates", ctypes.c_uint32), ("maxBatchCells", ctypes.c_uint64),
+                ("arenaBytes", ctypes.c_uint64)]
 
 
 class Request(ctypes.Structure):
@@ -88,7 +90,9 @@ class ServeStats(ctypes.Structure):
     _fields_ = [("structSize", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("requests", ctypes.c_uint64),
                 ("batches", ctypes.c_uint64), ("candidates", ctypes.c_uint64),
                 ("maxBatchRequests", ctypes.c_uint64), ("busyMs", ctypes.c_double),
-                ("admitMs", ctypes.c_double), ("completeMs", ctypes.c_double)]
+                ("admitMs", ctypes.c_double), ("completeMs", ctypes.c_double),
+                ("residentWorkgroups", ctypes.c_uint32), ("residentLdsBytes", ctypes.c_uint32),
+                ("residentItems", ctypes.c_uint64)]
 
 
 class SynthSpec(ctypes.Structure):
@@ -568,7 +572,7 @@ class Server:
     """Continuous-batching server over all buffers of a Myers instance."""
 
     def __init__(self, myers, distance=0.1, flags=0, hbm_budget=0, poll_us=0, max_batch_candidates=0,
-                 max_batch_cells=0):
+                 max_batch_cells=0, arena_bytes=0):
         cfg = ServeConfig()
         lib().myersServeDefaultConfig(ctypes.byref(cfg))
         cfg.distance = distance
@@ -580,6 +584,7 @@ class Server:
         cfg.maxBatchCandidates = max_batch_candidates
         if max_batch_cells:
             cfg.maxBatchCells = max_batch_cells
+        cfg.arenaBytes = arena_bytes
         self._h = ctypes.c_void_p()
         check(lib().myersServeStart(ctypes.byref(self._h), myers._handles, myers.num_buffers, ctypes.byref(cfg)),
               "myersServeStart")

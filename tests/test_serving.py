@@ -1,5 +1,8 @@
-"""Continuous-batching server: every request's results are the same alone and
-in the mix, and equal to the oracle; admission honours the HBM budget."""
+"""Serving: the continuous-batching server and the resident-kernel server.
+Every request's results are the same alone and in the mix, and equal to the
+oracle; admission honours the HBM budget."""
+import time
+
 import numpy as np
 import pytest
 
@@ -54,30 +57,69 @@ def oracle_req(ref, req):
     return O.myers_batch(ref, REF_SIZE, peq, qi, c, 0.1, O.MODE_ALIGNED)
 
 
-def test_results_same_alone_and_in_the_mix():
-    announce("serve mix invariance")
+MODES = [0, M.SERVE_RESIDENT]
+MODE_IDS = ["batches", "resident"]
+
+
+@pytest.mark.parametrize("serve", MODES, ids=MODE_IDS)
+def test_results_same_alone_and_in_the_mix(serve):
+    announce(f"serve mix invariance flags={serve}")
     ref, reqs = make_pool(41)
     with M.Myers(ref, REF_SIZE, num_buffers=3, avg_query=800, cands_per_query=11, buffer_bytes=256 << 20) as m:
         alone = [m.verify(*r, 0.1) for r in reqs]
-        with M.Server(m, distance=0.1) as sv:
+        with M.Server(m, distance=0.1, flags=serve) as sv:
             made = [M.Server.request(*r) for r in reqs]
             tickets = [sv.submit(x[0]) for x in made]
             for t in tickets:
                 assert sv.wait(t) >= 0.0
             st = sv.stats()
             assert st.requests == len(reqs)
-            assert st.maxBatchRequests >= 2          # requests really shared batches
+            if not serve:
+                assert st.maxBatchRequests >= 2      # requests really shared batches
+            else:
+                assert st.batches >= 1               # resident kernel launches
         for (req, keep), a, r in zip(made, alone, reqs):
             np.testing.assert_array_equal(keep[3], a)
             np.testing.assert_array_equal(a, oracle_req(ref, r))
 
 
-def test_open_loop_poisson():
-    announce("serve open loop")
+@pytest.mark.parametrize("mode", [M.WINDOW_ALIGNED, M.WINDOW_EXACT])
+def test_resident_window_modes_and_threshold(mode):
+    """The resident kernel in both window modes, and in threshold mode (the
+    cut-off kernels), against the oracle; the mix is submitted twice, the
+    second time after the kernel has left the device when idle.  The arena
+    (1 MB) wraps several times, so host memory the kernel read for earlier
+    requests is rewritten under it and read again."""
+    announce(f"serve resident mode={mode}")
+    ref, reqs = make_pool(45)
+    with M.Myers(ref, REF_SIZE, num_buffers=1, avg_query=800, cands_per_query=11, buffer_bytes=256 << 20) as m:
+        for e in (-1.0, 0.1):
+            m.set_threshold(e)
+            with M.Server(m, distance=0.1, flags=mode | M.SERVE_RESIDENT, arena_bytes=1 << 20) as sv:
+                for rnd in range(2):
+                    made = [M.Server.request(*r) for r in reqs]
+                    tickets = [sv.submit(x[0]) for x in made]
+                    for t in tickets:
+                        sv.wait(t)
+                    for (req, keep), r in zip(made, reqs):
+                        peq, qi, c = r
+                        full = O.myers_batch(ref, REF_SIZE, peq, qi, c, 0.1, mode)
+                        want = full if e < 0 else O.apply_threshold(qi, c, full, e)
+                        np.testing.assert_array_equal(keep[3], want, err_msg=f"round {rnd} e={e}")
+                    if rnd == 0:
+                        time.sleep(0.2)              # the kernel leaves after ~50 ms without work
+                launches = sv.stats().batches
+            assert launches >= 2
+        m.set_threshold(-1.0)
+
+
+@pytest.mark.parametrize("serve", MODES, ids=MODE_IDS)
+def test_open_loop_poisson(serve):
+    announce(f"serve open loop flags={serve}")
     ref, reqs = make_pool(42)
     rng = np.random.default_rng(5)
     with M.Myers(ref, REF_SIZE, num_buffers=3, avg_query=800, cands_per_query=11, buffer_bytes=256 << 20) as m:
-        with M.Server(m, distance=0.1) as sv:
+        with M.Server(m, distance=0.1, flags=serve) as sv:
             made = [M.Server.request(*r) for r in reqs * 3]
             gaps = rng.exponential(1e6 / 2000.0, size=len(made))          # 2000 requests/s
             arrivals = (np.cumsum(gaps) * 1e3).astype(np.uint64)
@@ -88,16 +130,18 @@ def test_open_loop_poisson():
             np.testing.assert_array_equal(keep[3], oracle_req(ref, r))
 
 
-def test_budget_limits_admission():
-    announce("serve budget")
+@pytest.mark.parametrize("serve", MODES, ids=MODE_IDS)
+def test_budget_limits_admission(serve):
+    announce(f"serve budget flags={serve}")
     ref, reqs = make_pool(43)
     with M.Myers(ref, REF_SIZE, num_buffers=2, avg_query=800, cands_per_query=11, buffer_bytes=256 << 20) as m:
-        with M.Server(m, distance=0.1, hbm_budget=1) as sv:     # one request in flight at a time
+        with M.Server(m, distance=0.1, hbm_budget=1, flags=serve) as sv:     # one request in flight at a time
             made = [M.Server.request(*r) for r in reqs[:10]]
             tickets = [sv.submit(x[0]) for x in made]
             for t in tickets:
                 sv.wait(t)
-            assert sv.stats().maxBatchRequests == 1
+            if not serve:
+                assert sv.stats().maxBatchRequests == 1
         for (req, keep), r in zip(made, reqs[:10]):
             np.testing.assert_array_equal(keep[3], oracle_req(ref, r))
 
@@ -116,3 +160,26 @@ def test_oversized_request_rejected():
             with pytest.raises(M.MyersError) as e:
                 sv.wait(t)
             assert e.value.code == 17
+
+
+def test_resident_oversized_request_rejected():
+    """A request larger than the resident server's arena fails with
+    MYERS_E_CAPACITY; the requests after it are served."""
+    announce("serve resident oversized")
+    ref, reqs = make_pool(46)
+    with M.Myers(ref, REF_SIZE, num_buffers=1, avg_query=800, cands_per_query=11, buffer_bytes=64 << 20) as m:
+        big = max(reqs, key=lambda r: len(r[2]))
+        small = min(reqs, key=lambda r: len(r[2]))
+        arena = 64 * 1024
+        need = sum(x.nbytes for x in big) + len(big[2]) * 12
+        assert need > arena
+        with M.Server(m, distance=0.1, flags=M.SERVE_RESIDENT, arena_bytes=arena) as sv:
+            req, keep = M.Server.request(*big)
+            t = sv.submit(req)
+            req2, keep2 = M.Server.request(*small)
+            t2 = sv.submit(req2)
+            with pytest.raises(M.MyersError) as e:
+                sv.wait(t)
+            assert e.value.code == 17
+            sv.wait(t2)
+        np.testing.assert_array_equal(keep2[3], oracle_req(ref, small))

@@ -565,8 +565,9 @@ def main():
     ap.add_argument("--serve-buffers", type=int, default=3)
     ap.add_argument("--serve-batch-cells", type=float, default=0, help="serving batch cap on cells (0 = library default)")
     ap.add_argument("--serve-engine", default="auto", choices=["auto", "plain", "persistent", "binned"])
-    ap.add_argument("--serve-mode", default="resident", choices=["resident", "batches"],
-                    help="serving by one resident kernel, or by continuous batching over the buffers")
+    ap.add_argument("--serve-mode", default="both", choices=["both", "resident", "batches"],
+                    help="serving by continuous batching over the buffers, by one resident kernel, or both "
+                         "(batching is the reported 'serving' line, the resident kernel 'serving_resident')")
     ap.add_argument("--variant", default=None, help="force a kernel variant for every read length, e.g. L2xW16")
     ap.add_argument("--threshold", type=float, default=0.2,
                     help="also time threshold mode at this error rate on config 2 (negative = skip)")
@@ -611,12 +612,15 @@ def main():
             c == 4 and not args.no_compare, args.cpu_seconds / 2 if c != 4 else args.cpu_seconds / 2,
             args.threshold if c == 2 else -1.0)
 
-    serving = None
-    if serve_loads:
-        log("serving ...")
+    serving = serving_resident = None
+    if serve_loads and args.serve_mode in ("both", "batches"):
+        log("serving (continuous batching) ...")
         serving = measure_serving(m, ref, cfg["ref_size"], serve_loads, args.serve_seconds,
-                                  int(args.serve_batch_cells), engines[args.serve_engine],
-                                  resident=args.serve_mode == "resident")
+                                  int(args.serve_batch_cells), engines[args.serve_engine], resident=False)
+    if serve_loads and args.serve_mode in ("both", "resident"):
+        log("serving (resident kernel) ...")
+        serving_resident = measure_serving(m, ref, cfg["ref_size"], serve_loads, args.serve_seconds,
+                                           int(args.serve_batch_cells), engines[args.serve_engine], resident=True)
 
     out = {
         "metric": "candidates/s & GCUPS on 1 MI355X (bit-exact); p99 latency under serving load",
@@ -646,6 +650,7 @@ def main():
         "cpu_baseline": head.get("cpu_baseline"),
         "legs": leg_out,
         "serving": serving,
+        "serving_resident": serving_resident,
     }
     print(json.dumps(out), flush=True)
     m.close()

@@ -161,8 +161,9 @@ int myers_launch_queue(const myers_verify_args_t *args, const myers_queue_plan_t
  *            counter; the wave that completes the request writes the
  *            request's ticket to flags[slot] after its results are visible.
  *   exit:    every wave leaves when *host_stop equals the launch's `gen` and
- *            no item is left, or when no item was claimed for idle_ticks
- *            (s_memrealtime ticks, 100 MHz); so every launch ends by itself.
+ *            no item is left, or after idle_ticks (its own s_memrealtime
+ *            ticks, 100 MHz) in which no ring's head moved; so every launch
+ *            ends by itself.
  * Item and request counters are u32 and compared modulo 2^32. */
 typedef struct {
     uint32_t key;          /* lanes << 8 | words of the chunk's variant */
@@ -187,11 +188,11 @@ typedef struct {
 } myers_serve_req_t;
 
 /* dev words: per ring r, head at MYERS_SERVE_HEAD + 64 r and device tail
- * at MYERS_SERVE_DTAIL + 64 r; then the poll time, stop generation and last
- * claim time.  simd_busy: MYERS_SERVE_SIMD_KEYS words. */
+ * at MYERS_SERVE_DTAIL + 64 r; then the poll time and the stop generation.
+ * simd_busy: MYERS_SERVE_SIMD_KEYS words. */
 enum { MYERS_SERVE_RINGS = 2, MYERS_SERVE_LONG_RING = 0, MYERS_SERVE_SHORT_RING = 1,
        MYERS_SERVE_HEAD = 0, MYERS_SERVE_DTAIL = 32, MYERS_SERVE_POLL = 128, MYERS_SERVE_STOP = 160,
-       MYERS_SERVE_LAST = 192, MYERS_SERVE_DEV_WORDS = 224, MYERS_SERVE_TAIL_STRIDE = 16,
+       MYERS_SERVE_DEV_WORDS = 192, MYERS_SERVE_TAIL_STRIDE = 16,
        MYERS_SERVE_SIMD_KEYS = 16384, MYERS_SERVE_LONG_LANES = 16 };
 
 typedef struct {

@@ -1281,20 +1281,21 @@ __device__ __forceinline__ bool serve_take(uint32_t *head, uint32_t *dtail, uint
 }
 
 // Lane 0: the next item's index in its ring (`ring`), or false when the wave
-// must exit.  A long item is taken only with the SIMD's flag, which the caller clears
-// when the item is done.  The idle time counts from the later of the grid's
-// last claim and this wave's start (the last claim may date from an earlier
-// launch).  An idle wave backs off from ~3 us to ~220 us between polls, so
-// that a grid of idle waves does not flood the memory system with atomics
-// while a few waves run long chains.  The wave's own count of idle polls
-// bounds the wait as well (under ~30 s), whatever the real-time counter does.
-__device__ bool serve_claim(const myers_serve_args_t &s, uint64_t t_start, uint32_t simd, uint32_t &item,
-                            uint32_t &ring) {
+// must exit.  A long item is taken only with the SIMD's count below
+// long_per_simd; the caller lowers it when the item is done.  The wave
+// leaves when the host's stop generation is seen, or after idle_ticks in
+// which no item of either ring was claimed anywhere (the rings' heads did
+// not move), measured on this wave's own real-time counter.  An idle wave
+// backs off from ~3 us to ~220 us between polls, so that a grid of idle
+// waves does not flood the memory system with atomics while a few waves run
+// long chains; its count of polls without progress bounds the wait as well
+// (under ~30 s), whatever the real-time counter does.
+__device__ bool serve_claim(const myers_serve_args_t &s, uint32_t simd, uint32_t &item, uint32_t &ring) {
     uint32_t *dstop = s.dev + MYERS_SERVE_STOP;
     uint64_t *poll = reinterpret_cast<uint64_t *>(s.dev + MYERS_SERVE_POLL);
-    uint64_t *last = reinterpret_cast<uint64_t *>(s.dev + MYERS_SERVE_LAST);
     uint32_t *busy = s.simd_busy + simd;
-    uint32_t idle = 0, polls = 0;
+    uint32_t idle = 0, polls = 0, heads = 0xFFFFFFFFu;
+    uint64_t t_idle = __builtin_amdgcn_s_memrealtime();
     for (;;) {
         // a long item, if fewer than long_per_simd long items run on this SIMD
         uint32_t *hl = s.dev + MYERS_SERVE_HEAD, *tl = s.dev + MYERS_SERVE_DTAIL;
@@ -1306,8 +1307,6 @@ __device__ bool serve_claim(const myers_serve_args_t &s, uint64_t t_start, uint3
                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                 if (serve_take(hl, tl, item)) {
                     ring = MYERS_SERVE_LONG_RING;
-                    __hip_atomic_store(last, __builtin_amdgcn_s_memrealtime(), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
                     return true;
                 }
                 __hip_atomic_fetch_sub(busy, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1315,11 +1314,20 @@ __device__ bool serve_claim(const myers_serve_args_t &s, uint64_t t_start, uint3
         }
         if (serve_take(s.dev + MYERS_SERVE_HEAD + 64u, s.dev + MYERS_SERVE_DTAIL + 64u, item)) {
             ring = MYERS_SERVE_SHORT_RING;
-            __hip_atomic_store(last, __builtin_amdgcn_s_memrealtime(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return true;
         }
-        // nothing claimable: refresh the mirrors of the host tails (one wave per poll period)
+        // nothing claimable here.  Progress anywhere (a head moved) restarts
+        // the idle time; then refresh the mirrors of the host tails (one wave
+        // per poll period).
         const uint64_t now = __builtin_amdgcn_s_memrealtime();
+        uint32_t sum = 0;
+        for (uint32_t r = 0; r < (uint32_t)MYERS_SERVE_RINGS; r++)
+            sum += __hip_atomic_load(s.dev + MYERS_SERVE_HEAD + 64u * r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (sum != heads) {
+            heads = sum;
+            t_idle = now;
+            polls = 0;
+        }
         uint64_t seen = __hip_atomic_load(poll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         polls++;
         if ((polls & 255u) == 0u ||
@@ -1347,9 +1355,7 @@ __device__ bool serve_claim(const myers_serve_args_t &s, uint64_t t_start, uint3
             }
         }
         if (__hip_atomic_load(dstop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == s.gen) return false;
-        uint64_t l = __hip_atomic_load(last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        l = l > t_start ? l : t_start;
-        if ((now > l && now - l > s.idle_ticks) || polls > (1u << 17)) return false;
+        if (now - t_idle > s.idle_ticks || polls > (1u << 17)) return false;
         idle = idle < 64u ? idle + 1u : 64u;
         for (uint32_t k = 0; k < idle; k++) __builtin_amdgcn_s_sleep(127);
     }
@@ -1382,11 +1388,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
 void verify_serve_kernel(myers_serve_args_t s) {
     extern __shared__ uint4 serve_lds[];
     uint32_t prepared = 0xFFFFFFFFu;
-    const uint64_t t_start = __builtin_amdgcn_s_memrealtime();
     const uint32_t simd = simd_key();
     for (;;) {
         uint32_t got = 0, got_ring = 2u;           // ring 2: exit
-        if (threadIdx.x == 0 && !serve_claim(s, t_start, simd, got, got_ring)) got_ring = 2u;
+        if (threadIdx.x == 0 && !serve_claim(s, simd, got, got_ring)) got_ring = 2u;
         const uint32_t ring = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)got_ring, 0));
         if (ring > 1u) break;
         const uint32_t item = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)got, 0));

@@ -183,3 +183,34 @@ def test_resident_oversized_request_rejected():
             assert e.value.code == 17
             sv.wait(t2)
         np.testing.assert_array_equal(keep2[3], oracle_req(ref, small))
+
+
+def test_resident_variant_boundaries():
+    """Reads at the capacities of the long-read variants the resident kernel
+    switches between (448, 512, 640 and 768 words), the longest supported
+    read, and short reads, in one request and in both window modes, against
+    the oracle."""
+    announce("serve resident boundaries")
+    ref = M.synth_reference(47, REF_SIZE)
+    lengths = [14336, 14337, 16384, 16385, 20480, 20481, 24575, 24576, 1000, 100]
+    parts = [M.synth_batch(M.make_spec(48 + k, 1, length=n, sub=0.04, ins=0.01, dele=0.01, candidates=3,
+                                       distance=0.1), ref, REF_SIZE) for k, n in enumerate(lengths)]
+    peq, qinfo, cands, e_off = [], [], [], 0
+    for k, b in enumerate(parts):
+        qi = b["qinfo"].copy()
+        qi[:, 0] += np.uint32(e_off)
+        c = b["cands"].copy()
+        c["query"] += np.uint32(k)
+        peq.append(b["peq"])
+        qinfo.append(qi)
+        cands.append(c)
+        e_off += len(b["peq"])
+    req = (np.concatenate(peq), np.concatenate(qinfo), np.concatenate(cands))
+    assert sorted(req[1][:, 1].tolist()) == sorted(lengths)
+    with M.Myers(ref, REF_SIZE, num_buffers=1, avg_query=8000, cands_per_query=3, buffer_bytes=256 << 20) as m:
+        for mode in (M.WINDOW_ALIGNED, M.WINDOW_EXACT):
+            with M.Server(m, distance=0.1, flags=mode | M.SERVE_RESIDENT) as sv:
+                r, keep = M.Server.request(*req)
+                sv.wait(sv.submit(r))
+            want = O.myers_batch(ref, REF_SIZE, req[0], req[1], req[2], 0.1, mode)
+            np.testing.assert_array_equal(keep[3], want)

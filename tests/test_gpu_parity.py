@@ -518,3 +518,46 @@ def test_queue_launch_every_variant():
         got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=flags)
         assert m.stats().launches == 2
         np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b, 0.1, O.MODE_ALIGNED))
+
+
+def test_reference_beyond_32_bit_positions():
+    """A 4.5 Gbase reference (loaded from memory with 64-bit sizes): random
+    loci over the whole reference, candidates placed around 2^31 and 2^32,
+    and the skip-rule edges at the end, through the plain, persistent and
+    per-bin engines in both window modes, against the oracle."""
+    size = 4_500_000_000
+    announce(f"reference of {size} bases")
+    ref = M.synth_reference(91, size)
+    assert len(ref) * 8 >= size
+    b = concat_batches([M.synth_batch(M.make_spec(92 + k, 40, length=n, sub=0.03, ins=0.005, dele=0.005,
+                                                  candidates=4, distance=0.1), ref, size)
+                        for k, n in enumerate([100, 1000, 5000])])
+    cands = b["cands"].copy()
+    m_of = b["qinfo"][cands["query"], 1].astype(np.int64)
+    n_of = np.array([O.load().oracle_window_len(int(x), 0.1) for x in m_of], dtype=np.int64)
+    rng = np.random.default_rng(93)
+    picks = rng.permutation(len(cands))
+    edges = []
+    for x in (2**31 - 3, 2**31, 2**31 + 5, 2**32 - 7, 2**32, 2**32 + 3):
+        edges += [x, x + 1]
+    k = 0
+    for x in edges:                               # around the 32-bit boundaries
+        cands["position"][picks[k]] = x
+        k += 1
+    for off in (1, 2, 0, -1, -5):                  # size - pos vs n: live for pos <= size - n - 1
+        i = picks[k]
+        cands["position"][i] = size - n_of[i] - off
+        k += 1
+    for pos in (size - 1, size, size + 10):        # beyond the end: skipped
+        cands["position"][picks[k]] = pos
+        k += 1
+    b = dict(b, cands=cands)
+    assert int(cands["position"].max()) >= 2**32 and (cands["position"] > 2**31).sum() > 20
+    with M.Myers(ref, size, avg_query=2000, cands_per_query=4, buffer_bytes=256 << 20) as m:
+        for mode in (M.WINDOW_ALIGNED, M.WINDOW_EXACT):
+            want = oracle_for(ref, size, b, 0.1, mode)
+            assert (want[:, 1] == 0xFFFFFFFF).sum() >= 5
+            for engine in (M.ENGINE_PLAIN, M.ENGINE_PERSISTENT, M.ENGINE_BINNED):
+                announce(f"beyond 32 bits mode={mode} engine={engine}")
+                got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=mode | engine)
+                np.testing.assert_array_equal(got, want)

@@ -1027,8 +1027,9 @@ __global__ __launch_bounds__(64 * Q) void verify_wide_persistent_kernel(myers_ve
 // 32*32 = 1024 bases in one lane; L > 1 covers long reads (up to 64*12*32 =
 // 24576 bases).
 #define MYERS_VARIANT_LIST(X)                                                                    \
-    X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 5) X(1, 6) X(1, 7) X(1, 8) X(1, 10) X(1, 12) X(1, 14)   \
-    X(1, 16) X(1, 20) X(1, 24) X(1, 28) X(1, 32)                                                 \
+    X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 5) X(1, 6) X(1, 7) X(1, 8) X(1, 9) X(1, 10) X(1, 11)     \
+    X(1, 12) X(1, 13) X(1, 14) X(1, 16) X(1, 19) X(1, 20) X(1, 22) X(1, 24) X(1, 25) X(1, 28)    \
+    X(1, 29) X(1, 32)                                                                            \
     X(2, 8) X(2, 12) X(2, 16) X(4, 8) X(4, 12) X(4, 16) X(8, 8) X(8, 12) X(8, 16)                \
     X(16, 8) X(16, 10) X(16, 12) X(16, 16) X(32, 7) X(32, 8) X(32, 10) X(32, 12) X(32, 16)        \
     X(64, 7) X(64, 8) X(64, 10) X(64, 12)

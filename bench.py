@@ -13,7 +13,7 @@ the kernel sized for the longest read.
 
 Beside the headline, by default: config 2 (100K x 1000 bp, plain one-length
 kernel) and config 4 (50K long noisy reads, 2-20 kb), each with its own
-roofline and a CPU check on a sample; threshold mode on config 2; the
+roofline and a CPU check on a sample; threshold mode on each; the
 serving sweep (config 5).
 
 A step is one pass of the pipeline (binning + verification) over the whole
@@ -570,7 +570,7 @@ def main():
                          "(batching is the reported 'serving' line, the resident kernel 'serving_resident')")
     ap.add_argument("--variant", default=None, help="force a kernel variant for every read length, e.g. L2xW16")
     ap.add_argument("--threshold", type=float, default=0.2,
-                    help="also time threshold mode at this error rate on config 2 (negative = skip)")
+                    help="also time threshold mode at this error rate on every measured config (negative = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (candidates would shard by read "
@@ -603,14 +603,14 @@ def main():
 
     head = measure_config(m, ref, args.config, args.steps, args.warmup, engines[args.engine], args.mode, args.graph,
                           not args.no_compare, args.cpu_seconds,
-                          args.threshold if args.config == 2 else -1.0)
+                          args.threshold)
     leg_out = {}
     for c in legs:
         steps = args.steps if c == 2 else max(3, args.steps // 4)
         leg_out[CONFIGS[c]["name"]] = measure_config(
             m, ref, c, steps, max(1, args.warmup // 2), engines[args.engine], args.mode, args.graph,
             c == 4 and not args.no_compare, args.cpu_seconds / 2 if c != 4 else args.cpu_seconds / 2,
-            args.threshold if c == 2 else -1.0)
+            args.threshold)
 
     serving = serving_resident = None
     if serve_loads and args.serve_mode in ("both", "batches"):
@@ -645,8 +645,8 @@ def main():
         },
         "roofline": head["roofline"],
         "engine_comparison": head.get("engine_comparison"),
-        "threshold_mode": head.get("threshold_mode") or next(
-            (x["threshold_mode"] for x in leg_out.values() if "threshold_mode" in x), None),
+        "threshold_mode": {name: x["threshold_mode"] for name, x in
+                           [(cfg["name"], head)] + list(leg_out.items()) if "threshold_mode" in x} or None,
         "cpu_baseline": head.get("cpu_baseline"),
         "legs": leg_out,
         "serving": serving,

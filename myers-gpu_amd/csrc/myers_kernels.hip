@@ -183,6 +183,95 @@ __device__ __forceinline__ uint32_t from_lower_lane(uint32_t v, uint32_t not_fir
     return v;
 }
 
+// One group of four words (g) of one column: the recurrence over words
+// 4g..4g+3 with the Eq words e (src/myers-4bits.c:115-134 block form).  carry
+// is the add's carry between words; ph_prev / mh_prev the Ph / Mh (before the
+// shift) of the word below, whose top bit shifts into this group's first word.
+//
+// kPhased: issue the group's word updates in phases (below).  With several
+// waves per SIMD this raises the share of quad-cycles that issue two VALU
+// instructions (18% -> 35% on 1000-base reads, 11% faster); a wave that runs
+// a long chain alone on its SIMD (the long-read launch, multi-wave
+// candidates) is 5-10% faster with the compiler's interleaved order, which
+// overlaps the carry chain with the other words' operations.
+template <int W, bool kHin, bool kPhased>
+__device__ __forceinline__ void group_update(int g, uint32_t (&Pv)[W], uint32_t (&Mv)[W], const uint4 &e,
+                                             uint32_t mh_in, uint32_t &carry, uint32_t &ph_prev,
+                                             uint32_t &mh_prev) {
+    const uint32_t e4[4] = {e.x, e.y, e.z, e.w};
+    if constexpr (kPhased) {
+        const int k0 = 4 * g;
+        const int n = (W - k0) < 4 ? (W - k0) : 4;
+        uint32_t eqv[4], t[4], xv[4], sum[4], xh[4], ph[4], mh[4], php[4], mhp[4];
+        // The group's words go through the recurrence in phases, each
+        // phase over all of its words, so that the double-rate bit
+        // operations of independent words issue back to back (a SIMD
+        // issues two of them in one quad-cycle) and the single-rate ones
+        // (add with carry, funnel shifts) come in runs:
+        //   Eq & Pv, Xv | carry chain | Xh, Ph, Mh | shifts | Pv, Mv.
+        // sched_barrier keeps the compiler from interleaving the phases.
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (j >= n) break;
+            const int k = k0 + j;
+            eqv[j] = (kHin && k == 0) ? (e4[0] | (mh_in >> 31)) : e4[j];
+            t[j] = eqv[j] & Pv[k];
+            xv[j] = e4[j] | Mv[k];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (j >= n) break;
+            uint32_t c_out;
+            sum[j] = __builtin_addc(t[j], Pv[k0 + j], carry, &c_out);
+            carry = c_out;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < 4 && j < n; j++) xh[j] = __builtin_amdgcn_bitop3_b32(sum[j], Pv[k0 + j], eqv[j], kXorOr);
+#pragma unroll
+        for (int j = 0; j < 4 && j < n; j++) {
+            ph[j] = __builtin_amdgcn_bitop3_b32(Mv[k0 + j], xh[j], Pv[k0 + j], kOrNor);
+            mh[j] = Pv[k0 + j] & xh[j];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < 4 && j < n; j++) {
+            const int k = k0 + j;
+            php[j] = (!kHin && k == 0) ? (ph[j] << 1) : __builtin_amdgcn_alignbit(ph[j], ph_prev, 31u);
+            mhp[j] = (!kHin && k == 0) ? (mh[j] << 1) : __builtin_amdgcn_alignbit(mh[j], mh_prev, 31u);
+            ph_prev = ph[j];
+            mh_prev = mh[j];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < 4 && j < n; j++) {
+            Pv[k0 + j] = __builtin_amdgcn_bitop3_b32(mhp[j], xv[j], php[j], kOrNor);
+            Mv[k0 + j] = php[j] & xv[j];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    } else {
+#pragma unroll
+        for (int k = 4 * g; k < 4 * g + 4 && k < W; k++) {
+            const uint32_t pv = Pv[k], mv = Mv[k];
+            const uint32_t xv = e4[k - 4 * g] | mv;
+            const uint32_t eq = (kHin && k == 0) ? (e4[0] | (mh_in >> 31)) : e4[k - 4 * g];
+            uint32_t c_out;
+            const uint32_t sum = __builtin_addc(eq & pv, pv, carry, &c_out);
+            carry = c_out;
+            const uint32_t xh = __builtin_amdgcn_bitop3_b32(sum, pv, eq, kXorOr);
+            const uint32_t ph = __builtin_amdgcn_bitop3_b32(mv, xh, pv, kOrNor);
+            const uint32_t mh = pv & xh;
+            const uint32_t php = (!kHin && k == 0) ? (ph << 1) : __builtin_amdgcn_alignbit(ph, ph_prev, 31u);
+            const uint32_t mhp = (!kHin && k == 0) ? (mh << 1) : __builtin_amdgcn_alignbit(mh, mh_prev, 31u);
+            ph_prev = ph;
+            mh_prev = mh;
+            Pv[k] = __builtin_amdgcn_bitop3_b32(mhp, xv, php, kOrNor);
+            Mv[k] = php & xv;
+        }
+    }
+}
+
 // One column for one lane's W words.  Eq words stream from LDS one group of
 // four words ahead of use (group g+1 is read while group g computes; the
 // caller reads the next column's group 0 at the start of the column), so 12
@@ -190,29 +279,20 @@ __device__ __forceinline__ uint32_t from_lower_lane(uint32_t v, uint32_t not_fir
 // lower lane's hin in bit 31 (src/myers-4bits.c:115-134 block form).  The
 // top word's Ph / Mh (before the shift) come back in ph_top / mh_top.
 //
-// kCut (L = 1): Ukkonen cut-off (Myers 1999, blocks model; GEMTools
-// gt_map_align_bpm.c:141-214): only the first ga groups are advanced.  ga is
-// the same in every lane, so a skipped group costs no issue slots; the group
-// test is made on a VGPR copy of ga (ga_v), which compiles to an exec mask
-// and a skip branch that keep every word in its register.  ph_top / mh_top
-// are then those of group ga-1's top word.
-//
-// kPhased: issue each group's word updates in phases (below).  With several
-// waves per SIMD this raises the share of quad-cycles that issue two VALU
-// instructions (18% -> 35% on 1000-base reads, 11% faster); a wave that runs
-// a long chain alone on its SIMD (the long-read launch, multi-wave
-// candidates) is 5-10% faster with the compiler's interleaved order, which
-// overlaps the carry chain with the other words' operations.
-template <int W, bool kHin, bool kCut, uint32_t kGS, bool kPhased = true>
+// kPhased: as in group_update.  The body repeats group_update's rather than
+// calling it: through the call the compiler sinks Xv into the last phase,
+// which measured 2% slower here, while pinning Xv in the first phase makes
+// the cut-off column (which calls group_update) spill.
+template <int W, bool kHin, uint32_t kGS, bool kPhased = true>
 __device__ __forceinline__ void column_grp(uint32_t (&Pv)[W], uint32_t (&Mv)[W], const uint4 &e0,
-                                           const uint4 *lds_lane, uint32_t cur_off, uint32_t ga_v,
-                                           uint32_t ph_in, uint32_t mh_in, uint32_t &ph_top, uint32_t &mh_top) {
+                                           const uint4 *lds_lane, uint32_t cur_off, uint32_t ph_in,
+                                           uint32_t mh_in, uint32_t &ph_top, uint32_t &mh_top) {
     constexpr int kG = (W + 3) / 4;     // kGS: uint4 between groups in LDS
     uint32_t carry = 0, ph_prev = ph_in, mh_prev = mh_in;
     uint4 cur = e0, nxt = e0;
 #pragma unroll
     for (int g = 0; g < kG; g++) {
-        if (!kCut || g == 0 || (uint32_t)g < ga_v) {
+        {
             if (g + 1 < kG) nxt = lds_lane[cur_off + (uint32_t)(g + 1) * kGS];
             const uint32_t e4[4] = {cur.x, cur.y, cur.z, cur.w};
             if constexpr (kPhased) {
@@ -293,6 +373,32 @@ __device__ __forceinline__ void column_grp(uint32_t (&Pv)[W], uint32_t (&Mv)[W],
     mh_top = mh_prev;
 }
 
+// The same column (L = 1) with the Ukkonen cut-off of threshold mode (Myers
+// 1999, blocks model; GEMTools gt_map_align_bpm.c:141-214): only groups
+// 0..ga-1 are advanced, ga being the same in every lane of the wave.  Group
+// G+1 runs inside group G's (scalar) branch, so a skipped group costs a
+// compare and a branch, the carry stays in its scalar register, and no
+// value needs merging between groups; ph_top / mh_top are those of group
+// ga-1's top word.
+template <int G, int W, uint32_t kGS>
+__device__ __forceinline__ void column_cut(uint32_t (&Pv)[W], uint32_t (&Mv)[W], const uint4 &cur,
+                                           const uint4 *lds_lane, uint32_t cur_off, uint32_t ga,
+                                           uint32_t carry, uint32_t ph_prev, uint32_t mh_prev,
+                                           uint32_t &ph_top, uint32_t &mh_top) {
+    constexpr int kG = (W + 3) / 4;
+    uint4 nxt = cur;
+    if constexpr (G + 1 < kG) nxt = lds_lane[cur_off + (uint32_t)(G + 1) * kGS];
+    group_update<W, false, true>(G, Pv, Mv, cur, 0u, carry, ph_prev, mh_prev);
+    if constexpr (G + 1 < kG) {
+        if ((uint32_t)(G + 1) < ga) {
+            column_cut<G + 1, W, kGS>(Pv, Mv, nxt, lds_lane, cur_off, ga, carry, ph_prev, mh_prev, ph_top, mh_top);
+            return;
+        }
+    }
+    ph_top = ph_prev;
+    mh_top = mh_prev;
+}
+
 template <int L, int W>
 struct Workspace {
     uint4 lds[Shape<L, W>::kLdsU4];
@@ -340,7 +446,7 @@ __device__ __forceinline__ void zero_row_init(Workspace<L, W> &ws) {
 
 // Verify chunks chunk_lo, chunk_lo + stride, ... < chunk_hi of the slot range
 // [start, start + count); a chunk is the kSlots candidates of one wave.
-// kCut: threshold mode.  L = 1 (W % 4 == 0): the Ukkonen cut-off; L > 1: a
+// kCut: threshold mode.  L = 1 (W >= 8): the Ukkonen cut-off; L > 1: a
 // wave stops once its candidates cannot get under the threshold.  kHost: the
 // batch's arrays are host memory of the resident server (ld32 / st_result).
 template <int L, int W, bool kCut, int kWaves = 1, bool kHost = false>
@@ -450,15 +556,17 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
             uint32_t ph_top = 0, mh_top = 0;
             // Cut-off state: rows with D <= k (k + pad in layout rows, where
             // the pad rows below the query hold D = row) must lie in the
-            // first ga groups, 8 rows below its top.  Start with every lane's
-            // rows 0..kv+8 covered; D at the top row R = 128 ga is then R.
-            constexpr uint32_t kG = (uint32_t)(W / 4);
+            // first ga groups, 8 rows below its top.  Groups hold 128 rows,
+            // the top one 32 (W - 4 (kG - 1)).  Start with every lane's rows
+            // 0..kv+8 covered; D at the top row R of group ga-1 is then R.
+            constexpr uint32_t kG = (uint32_t)((W + 3) / 4);
+            constexpr int32_t kTopRows = 32 * (W - 4 * ((W + 3) / 4 - 1));
             int32_t kv = 0;
             uint32_t ga = kG;
             if constexpr (kUkk) {
                 kv = (int32_t)threshold_of(m, a.threshold) + (32 * W - (int32_t)m);
                 ga = min(wave_max(ncols > 0u ? ((uint32_t)kv + 135u) / 128u : 1u), kG);
-                score = 128 * (int32_t)ga;
+                score = ga == kG ? 32 * W : 128 * (int32_t)ga;
             }
 
             // Text words stream through registers two blocks (16 columns) ahead
@@ -495,7 +603,8 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
                     // the last such row of column j - 1, so the extent must keep
                     // 8 rows of headroom over it: rows R-7..R with D <= k force
                     // D(R) <= k + 7 (one group is added), and a group can go
-                    // when its rows and the 8 below it exceed k: D(R) > k + 135.
+                    // when its rows and the 8 below it exceed k: D(R) > k + 135
+                    // (for the shorter top group a stricter test than needed).
                     const bool on = 8u * b < ncols;
                     if (ga < kG && __ballot(on && score <= kv + 7) != 0ull) {
                         // new group: D rises by one per row above R (Pv = ~0, Mv = 0)
@@ -503,12 +612,12 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
                         for (int g = 1; g < (int)kG; g++) {
                             const bool sel = (uint32_t)g == ga;
 #pragma unroll
-                            for (int k = 4 * g; k < 4 * g + 4; k++) {
+                            for (int k = 4 * g; k < 4 * g + 4 && k < W; k++) {
                                 Pv[k] = sel ? ~0u : Pv[k];
                                 Mv[k] = sel ? 0u : Mv[k];
                             }
                         }
-                        score += 128;
+                        score += ga + 1u == kG ? kTopRows : 128;
                         ga++;
                     } else if (ga > 1u && __ballot(on && score <= kv + 135) == 0ull) {
                         // drop group ga-1: S moves down by its vertical deltas
@@ -517,7 +626,7 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
                             if ((uint32_t)g + 1u == ga) {
                                 int32_t dv = 0;
 #pragma unroll
-                                for (int k = 4 * g; k < 4 * g + 4; k++)
+                                for (int k = 4 * g; k < 4 * g + 4 && k < W; k++)
                                     dv += __builtin_popcount(Pv[k]) - __builtin_popcount(Mv[k]);
                                 score -= dv;
                             }
@@ -525,16 +634,20 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
                         ga--;
                     }
                 }
-                uint32_t ga_v = ga;
-                if constexpr (kUkk) asm volatile("v_mov_b32 %0, %1" : "=v"(ga_v) : "s"(ga));
+                const uint32_t ga_s = __builtin_amdgcn_readfirstlane(ga);
 #pragma unroll
                 for (int i = 0; i < 8; i++) {
                     const uint32_t cn = (i < 7) ? ((text >> (4 * (i + 1))) & 7u) : (text_next & 7u);
                     const uint32_t next_off = cn * S::kRowU4;
                     if constexpr (L == 1) {
                         const uint4 eq4n = lds_lane[next_off];        // next column, group 0
-                        column_grp<W, false, kUkk, (uint32_t)S::kGroupU4>(
-                            Pv, Mv, eq4, lds_lane, cur_off, ga_v, 0u, 0u, ph_top, mh_top);
+                        if constexpr (kUkk) {
+                            column_cut<0, W, (uint32_t)S::kGroupU4>(Pv, Mv, eq4, lds_lane, cur_off, ga_s, 0u, 0u,
+                                                                   0u, ph_top, mh_top);
+                        } else {
+                            column_grp<W, false, (uint32_t)S::kGroupU4>(Pv, Mv, eq4, lds_lane, cur_off, 0u, 0u,
+                                                                       ph_top, mh_top);
+                        }
                         eq4 = eq4n;
                         score += (int32_t)(ph_top >> 31) - (int32_t)(mh_top >> 31);
                         const uint32_t col = 8u * b + (uint32_t)i;
@@ -558,8 +671,8 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
                         const uint4 eq4n = lds_lane[next_off];        // next column, group 0
                         const uint32_t ph_in = from_lower_lane<L>(ph_top, not_first);
                         const uint32_t mh_in = from_lower_lane<L>(mh_top, not_first);
-                        column_grp<W, true, false, (uint32_t)S::kGroupU4, kWaves == 1>(
-                            Pv, Mv, eq4, lds_lane, cur_off, 0u, ph_in, mh_in, ph_top, mh_top);
+                        column_grp<W, true, (uint32_t)S::kGroupU4, kWaves == 1>(
+                            Pv, Mv, eq4, lds_lane, cur_off, ph_in, mh_in, ph_top, mh_top);
                         eq4 = eq4n;
                         // score row and first minimum (src-api/coreMyersGPU-CC35.cu:205-207,227-228);
                         // only the group's top lane keeps them, at column t - (L-1)
@@ -644,12 +757,12 @@ __device__ __forceinline__ bool pull_work(uint32_t *heads, uint32_t n, uint32_t 
 // kernels for W >= 28 then spill a few per-chunk values, outside the column
 // loop.
 template <int L, int W, bool kCut, bool kPersistent>
-constexpr int kWavesPerEU = L != 1 ? 1 : (W >= 28 ? 3 : ((W >= 20 && !kPersistent && !kCut) ? 4 : 1));
+constexpr int kWavesPerEU = L != 1 ? 1 : (W >= 28 ? 3 : (W >= 20 ? ((!kPersistent && !kCut) ? 4 : 3) : 1));
 
-// Variants with a threshold-mode kernel: the cut-off (L = 1, W % 4 == 0,
-// W >= 8) or the early stop (multi-lane).
+// Variants with a threshold-mode kernel: the cut-off (L = 1, W >= 8) or the
+// early stop (multi-lane).
 template <int L, int W>
-constexpr bool kHasCut = (L == 1 && W >= 8 && W % 4 == 0) || (L > 1 && L <= 64);
+constexpr bool kHasCut = (L == 1 && W >= 8) || (L > 1 && L <= 64);
 
 // Issue priority of a variant's waves (s_setprio).  A wave of a long read
 // runs one candidate's chain of dependent column steps, and the chain of the
@@ -858,7 +971,7 @@ __device__ __forceinline__ uint2 wide_columns(WideWorkspace<Q, W> &ws, uint32_t 
             }
             const uint32_t ph_in = (uint32_t)__builtin_amdgcn_update_dpp((int)fill_p, (int)ph_top, 0x138, 0xF, 0xF, false);
             const uint32_t mh_in = (uint32_t)__builtin_amdgcn_update_dpp((int)fill_m, (int)mh_top, 0x138, 0xF, 0xF, false);
-            column_grp<W, true, false, kGS, false>(Pv, Mv, eq4, lds_lane, cur_off, 0u, ph_in, mh_in, ph_top, mh_top);
+            column_grp<W, true, kGS, false>(Pv, Mv, eq4, lds_lane, cur_off, ph_in, mh_in, ph_top, mh_top);
             eq4 = eq4n;
             if constexpr (kAbove) {
                 acc_p = (acc_p >> 1) | (ph_top & 0x80000000u);

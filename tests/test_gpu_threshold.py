@@ -26,9 +26,10 @@ def expected(ref, size, b, distance, mode, e):
 
 
 def test_threshold_every_single_lane_variant():
-    """Every one-lane variant (cut-off kernels for W = 8..32 in steps of 4,
-    post-filtered full kernels for the others), lengths across the variant's
-    range in one wave, error rates around the thresholds."""
+    """Every one-lane variant (cut-off kernels for W >= 8, among them the odd
+    W whose top group holds fewer than four words; post-filtered full kernels
+    for W < 8), lengths across the variant's top four words in one wave,
+    error rates around the thresholds."""
     size = 2_000_000
     ref = M.synth_reference(61, size)
     with M.Myers(ref, size, avg_query=600, cands_per_query=6, buffer_bytes=256 << 20) as m:

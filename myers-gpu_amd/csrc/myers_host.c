@@ -740,7 +740,7 @@ static uint32_t long_plan(const myers_ctx_t *ctx, const bin_scan_t *scan, const 
     return nt;
 }
 
-/* Resident one-wave workgroups per CU of a queue kernel with `lds` bytes of
+/* Resident workgroups per CU of a queue kernel with `lds` bytes of
  * dynamic LDS (queried once per size). */
 static uint32_t queue_grid(myers_ctx_t *ctx, uint32_t cls, int cut, uint32_t lds)
 {

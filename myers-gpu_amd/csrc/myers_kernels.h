@@ -109,16 +109,19 @@ int myers_variant_long_ok(uint32_t v, uint32_t *ws_u4);
 int myers_launch_long(const myers_verify_args_t *args, const myers_long_plan_t *plan, uint32_t grid,
                       hipStream_t stream);
 
-/* One launch over all length bins ("queue launch").  A resident grid of
- * one-wave workgroups pulls chunks from one device counter over the plan's
- * bins in plan order (longest reads first, so the tail is short chunks);
- * each chunk runs the variant of its bin through a switch over the compiled
- * variants.  Variants fall into two kernel classes with their own register
- * budgets: MYERS_QUEUE_ONE_LANE (L = 1) and MYERS_QUEUE_MULTI_LANE (L = 2..64
- * with W <= 12); other variants keep per-bin launches.  The class's counter
- * (queue_heads + MYERS_QUEUE_HEAD(cls)) is reset by the binning.  LDS per
- * workgroup: ws_u4 x 16 bytes (the largest workspace of the plan's
- * variants).  Needs bin_ranges and queue_heads. */
+/* One launch over all length bins ("queue launch").  A resident grid pulls
+ * chunks from one device counter over the plan's bins in plan order (longest
+ * reads first, so the tail is short chunks); each chunk runs the variant of
+ * its bin through a switch over the compiled variants.  Variants fall into
+ * two kernel classes with their own register budgets: MYERS_QUEUE_ONE_LANE
+ * (L = 1; one-wave workgroups, one chunk per item) and MYERS_QUEUE_MULTI_LANE
+ * (L = 2..64 with W <= 12; workgroups of myers_queue_waves() = 2 waves that
+ * take a pair of chunks per item and share one staged Peq, so that each wave
+ * needs half the LDS); other variants keep per-bin launches.  The class's
+ * counter (queue_heads + MYERS_QUEUE_HEAD(cls)) is reset by the binning.
+ * LDS per workgroup: ws_u4 x 16 bytes (the largest workspace of the plan's
+ * variants, as myers_variant_queue_class reports it).  Needs bin_ranges and
+ * queue_heads. */
 enum { MYERS_QUEUE_ONE_LANE = 0, MYERS_QUEUE_MULTI_LANE = 1, MYERS_QUEUE_CLASSES = 2, MYERS_QUEUE_MAX = 48 };
 #define MYERS_QUEUE_HEAD(cls) (MYERS_LONG_HEAD + (2u + (uint32_t)(cls)) * MYERS_QUEUE_STRIDE)
 typedef struct {
@@ -134,7 +137,9 @@ int myers_variant_queue_class(uint32_t v, uint32_t *ws_u4);
 /* Resident workgroups per CU of a class's queue kernel with `lds_bytes` of
  * dynamic LDS (cut: the threshold-mode kernel). */
 int myers_queue_occupancy(uint32_t cls, int cut, uint32_t lds_bytes, int *wg_per_cu);
-/* grid = one-wave workgroups (normally CUs x occupancy). */
+/* Waves per workgroup of a class's queue kernel. */
+uint32_t myers_queue_waves(uint32_t cls);
+/* grid = workgroups (normally CUs x occupancy). */
 int myers_launch_queue(const myers_verify_args_t *args, const myers_queue_plan_t *plan, uint32_t grid,
                        hipStream_t stream);
 

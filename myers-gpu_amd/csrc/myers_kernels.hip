@@ -136,13 +136,13 @@ __device__ __forceinline__ void st_result(uint2 *p, uint2 v) {
 // j (rows 32j..32j+31) of base c is bitmap[c][j%4] of entry posEntry + j/4;
 // words outside the query read 0.  Lane gl = gw / W of the group reads word
 // k = gw % W at LDS uint4 (k/4)*kGroupU4 + slot*kSlotU4 + c*L + gl, lane k%4.
-template <int L, int W, bool kHost = false>
+template <int L, int W, bool kHost = false, uint32_t kThreads = 64>
 __device__ __forceinline__ void stage_peq(uint4 *lds, const uint2 *qtab, uint32_t nslots,
                                           const uint32_t *__restrict__ peq, uint32_t lane) {
     using S = Shape<L, W>;
     uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
     const uint32_t total = nslots * 5u * (uint32_t)S::kTotal;
-    for (uint32_t idx = lane; idx < total; idx += 64u) {
+    for (uint32_t idx = lane; idx < total; idx += kThreads) {
         const uint32_t gw = idx % (uint32_t)S::kTotal;
         const uint32_t rest = idx / (uint32_t)S::kTotal;
         const uint32_t c = rest % 5u;
@@ -428,14 +428,17 @@ __device__ __forceinline__ uint32_t wave_lane() {
     return kWaves == 1 ? threadIdx.x : (threadIdx.x & 63u);
 }
 
-template <int L, int W, int kWaves = 1>
+// kThreads: the threads sharing the workspace (64, or 128 for a workgroup of
+// two waves that stage together).
+template <int L, int W, int kWaves = 1, uint32_t kThreads = 64>
 __device__ __forceinline__ void zero_row_init(Workspace<L, W> &ws) {
     using S = Shape<L, W>;
     if constexpr (S::kRows > 5) {
         // the all-zero row read by lanes before their first column
         uint32_t *l32 = reinterpret_cast<uint32_t *>(ws.lds);
         constexpr uint32_t kPerGroup = (uint32_t)(S::kQSlots * L);    // (slot, lane) pairs
-        for (uint32_t i = wave_lane<kWaves>(); i < (uint32_t)S::kGroups * kPerGroup * 4u; i += 64u) {
+        const uint32_t t0 = kThreads == 64u ? wave_lane<kWaves>() : threadIdx.x;
+        for (uint32_t i = t0; i < (uint32_t)S::kGroups * kPerGroup * 4u; i += kThreads) {
             const uint32_t u4 = i >> 2, g = u4 / kPerGroup, r = u4 % kPerGroup;
             const uint32_t slot = r / (uint32_t)L, gl = r % (uint32_t)L;
             l32[(g * (uint32_t)S::kGroupU4 + slot * (uint32_t)S::kSlotU4 + kZeroRow * (uint32_t)L + gl) * 4u +
@@ -449,16 +452,24 @@ __device__ __forceinline__ void zero_row_init(Workspace<L, W> &ws) {
 // kCut: threshold mode.  L = 1 (W >= 8): the Ukkonen cut-off; L > 1: a
 // wave stops once its candidates cannot get under the threshold.  kHost: the
 // batch's arrays are host memory of the resident server (ld32 / st_result).
-template <int L, int W, bool kCut, int kWaves = 1, bool kHost = false>
+// kShare = 2: a workgroup of two waves verifies chunk pairs (2c, 2c+1) with
+// one workspace: the distinct reads of both waves' candidates share its
+// query slots (one pass while they number at most kQSlots), so each wave
+// needs half the LDS of a wave of its own.  xchg: three words of LDS for the
+// hand-over between the waves (the last read of wave 0, each wave's count of
+// reads).
+template <int L, int W, bool kCut, int kWaves = 1, bool kHost = false, int kShare = 1>
 __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint32_t start, uint32_t count,
                                               uint32_t chunk_lo, uint32_t chunk_hi, uint32_t stride,
-                                              Workspace<L, W> &ws) {
+                                              Workspace<L, W> &ws, uint32_t *xchg = nullptr) {
+    static_assert(kShare == 1 || (kShare == 2 && kWaves == 1 && !kHost), "shared workspace: one-wave chunks");
     using S = Shape<L, W>;
     constexpr bool kUkk = kCut && L == 1;       // the cut-off
     constexpr bool kStop = kCut && L > 1;       // the early stop
     uint4 *lds = ws.lds;
     uint2 *qtab = ws.qtab;
-    const uint32_t lane = wave_lane<kWaves>();
+    const uint32_t lane = kShare == 1 ? wave_lane<kWaves>() : (threadIdx.x & 63u);
+    const uint32_t wave = kShare == 1 ? 0u : (threadIdx.x >> 6);
     const uint32_t gl = lane % L;                      // lane within the candidate's group
     const uint32_t *__restrict__ ref = a.ref;
     const int32_t last_word = (int32_t)(a.ref_words - 1);
@@ -469,7 +480,7 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
     const uint64_t lanes_upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
 
     for (uint32_t chunk = chunk_lo; chunk < chunk_hi; chunk += stride) {
-        const uint32_t slot = chunk * S::kSlots + lane / L;
+        const uint32_t slot = (chunk * (uint32_t)kShare + wave) * S::kSlots + lane / L;
         const bool valid = slot < count;
         uint32_t cand = 0, m = 0, pos_entry = 0, query = 0xFFFFFFFFu;
         uint64_t pos = 0;
@@ -513,11 +524,25 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
 
         // Query slots: consecutive candidates of one query share a slot (the
         // callers' batches and the binning keep a query's candidates adjacent).
-        const uint32_t query_below = (uint32_t)__shfl_up((int)query, L);
-        const bool head = (lane < (uint32_t)L || query_below != query) && gl == 0u;
+        uint32_t query_below = (uint32_t)__shfl_up((int)query, L);
+        if constexpr (kShare == 2) {
+            // wave 1's first candidate continues wave 0's last read or heads a new one
+            if (wave == 0u && lane == 63u) xchg[0] = query;
+            __syncthreads();
+            if (wave == 1u && lane < (uint32_t)L) query_below = xchg[0];
+        }
+        const bool first = kShare == 1 ? lane < (uint32_t)L : (wave == 0u && lane < (uint32_t)L);
+        const bool head = (first || query_below != query) && gl == 0u;
         const uint64_t heads = __ballot(head);
-        const uint32_t qslot = (uint32_t)__builtin_popcountll(heads & lanes_upto) - 1u;
-        const uint32_t nqslots = (uint32_t)__builtin_popcountll(heads);
+        uint32_t slot_base = 0u, nqslots = (uint32_t)__builtin_popcountll(heads);
+        if constexpr (kShare == 2) {
+            if (lane == 0u) xchg[1u + wave] = nqslots;
+            __syncthreads();
+            const uint32_t h0 = xchg[1], h1 = xchg[2];
+            slot_base = wave == 0u ? 0u : h0;
+            nqslots = h0 + h1;
+        }
+        const uint32_t qslot = slot_base + (uint32_t)__builtin_popcountll(heads & lanes_upto) - 1u;
 
         // More distinct queries than LDS slots: one pass per kQSlots queries.
         for (uint32_t base_slot = 0; base_slot < nqslots; base_slot += S::kQSlots) {
@@ -526,7 +551,8 @@ __device__ __forceinline__ void verify_chunks(const myers_verify_args_t &a, uint
             chunk_sync<kWaves>();                        // previous readers of qtab / lds are done
             if (in_pass && head) qtab[my_slot] = make_uint2(pos_entry, m);
             chunk_sync<kWaves>();
-            stage_peq<L, W, kHost>(lds, qtab, min(nqslots - base_slot, (uint32_t)S::kQSlots), a.peq, lane);
+            stage_peq<L, W, kHost, 64u * kShare>(lds, qtab, min(nqslots - base_slot, (uint32_t)S::kQSlots), a.peq,
+                                                 kShare == 1 ? lane : threadIdx.x);
             chunk_sync<kWaves>();
             const uint4 *lds_lane = lds + my_slot * (uint32_t)S::kSlotU4 + gl;
             const uint32_t ncols = in_pass ? ncols_live : 0u;
@@ -1283,36 +1309,55 @@ __global__ __launch_bounds__(64 * MYERS_LONG_WAVES) void verify_long_kernel(myer
 template <int L, int W>
 constexpr int kQueueClass = L == 1 ? MYERS_QUEUE_ONE_LANE : ((L <= 64 && W <= 12) ? MYERS_QUEUE_MULTI_LANE : -1);
 
+// Waves per workgroup of a class's queue kernel.
+template <int kClass>
+constexpr int kQueueWaves = kClass == MYERS_QUEUE_MULTI_LANE ? 2 : 1;
+
 #define MYERS_QUEUE_CASE(L, W)                                                                   \
     case ((L) << 8) | (W):                                                                       \
         if constexpr (kQueueClass<(L), (W)> == kClass) {                                         \
-            auto &w = *reinterpret_cast<Workspace<(L), (W)> *>(queue_lds);                       \
+            auto &w = *reinterpret_cast<Workspace<(L), (W)> *>(queue_lds + kXchgU4);             \
             if (prepared != key) {                                                               \
                 set_wave_prio<(L)>();                                                            \
-                zero_row_init<(L), (W)>(w);                                                      \
+                zero_row_init<(L), (W), 1, 64u * kShare>(w);                                     \
                 prepared = key;                                                                  \
             }                                                                                    \
-            verify_chunks<(L), (W), kCut && kHasCut<(L), (W)>, 1>(a, start, count, chunk, chunk + 1u, 1u, w); \
+            verify_chunks<(L), (W), kCut && kHasCut<(L), (W)>, 1, false, kShare>(a, start, count, chunk, \
+                                                                                 chunk + 1u, 1u, w, xchg); \
         }                                                                                        \
         break;
 
 template <int kClass, bool kCut>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ __launch_bounds__(64 * kQueueWaves<kClass>) __attribute__((amdgpu_waves_per_eu(3)))
 void verify_queue_kernel(myers_verify_args_t a, myers_queue_plan_t p) {
     extern __shared__ uint4 queue_lds[];
+    // Multi-lane class: workgroups of two waves that take chunk pairs and
+    // share one workspace (verify_chunks, kShare = 2), after one uint4 of
+    // hand-over words; the item counter then counts chunk pairs.
+    constexpr int kShare = kQueueWaves<kClass>;
+    constexpr uint32_t kXchgU4 = kShare == 2 ? 1u : 0u;
+    uint32_t *xchg = reinterpret_cast<uint32_t *>(queue_lds);
     uint32_t *counter = a.queue_heads + MYERS_QUEUE_HEAD(kClass);
     uint32_t prepared = 0xFFFFFFFFu;
     for (;;) {
-        uint32_t got = 0;
-        if (threadIdx.x == 0) got = atomicAdd(counter, 1u);
-        const uint32_t item = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)got, 0));
+        uint32_t item;
+        if constexpr (kShare == 1) {
+            uint32_t got = 0;
+            if (threadIdx.x == 0) got = atomicAdd(counter, 1u);
+            item = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)got, 0));
+        } else {
+            if (threadIdx.x == 0) xchg[3] = atomicAdd(counter, 1u);
+            __syncthreads();
+            item = __builtin_amdgcn_readfirstlane(xchg[3]);
+            __syncthreads();                     // read by all before the next claim
+        }
         uint32_t j = 0, base = 0, start = 0, count = 0;
         bool found = false;
         for (; j < p.n; j++) {
             const uint32_t v = p.variant[j];
             const uint32_t c = a.bin_ranges[2u * v + 1u];
             const uint32_t slots = 64u / (p.key[j] >> 8);
-            const uint32_t chunks = (c + slots - 1u) / slots;
+            const uint32_t chunks = (c + slots * (uint32_t)kShare - 1u) / (slots * (uint32_t)kShare);
             if (item - base < chunks) {
                 start = a.bin_ranges[2u * v];
                 count = c;
@@ -1826,15 +1871,21 @@ const void *queue_kernel(uint32_t cls, bool cut) {
 extern "C" int myers_variant_queue_class(uint32_t v, uint32_t *ws_u4) {
     constexpr uint32_t kOneWave = sizeof(kQueueClassOf) / sizeof(kQueueClassOf[0]);
     if (v >= kOneWave || kQueueClassOf[v] < 0) return -1;
-    if (ws_u4) *ws_u4 = kWorkspaceU4[v];
+    // the multi-lane class's workgroups keep one uint4 of hand-over words first
+    if (ws_u4) *ws_u4 = kWorkspaceU4[v] + (kQueueClassOf[v] == MYERS_QUEUE_MULTI_LANE ? 1u : 0u);
     return kQueueClassOf[v];
+}
+
+extern "C" uint32_t myers_queue_waves(uint32_t cls) {
+    return cls == (uint32_t)MYERS_QUEUE_MULTI_LANE ? (uint32_t)kQueueWaves<MYERS_QUEUE_MULTI_LANE>
+                                                   : (uint32_t)kQueueWaves<MYERS_QUEUE_ONE_LANE>;
 }
 
 extern "C" int myers_queue_occupancy(uint32_t cls, int cut, uint32_t lds_bytes, int *wg_per_cu) {
     if (cls >= (uint32_t)MYERS_QUEUE_CLASSES || wg_per_cu == nullptr) return -1;
     int blocks = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, queue_kernel(cls, cut != 0), 64,
-                                                                (size_t)lds_bytes);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, queue_kernel(cls, cut != 0),
+                                                                64 * (int)myers_queue_waves(cls), (size_t)lds_bytes);
     if (e != hipSuccess) return (int)e;
     *wg_per_cu = blocks;
     return 0;
@@ -1857,8 +1908,8 @@ extern "C" int myers_launch_queue(const myers_verify_args_t *args, const myers_q
     }
     if (plan->ws_u4 < need || plan->ws_u4 * 16u > 64u * 1024u) return -1;
     void *kargs[] = {const_cast<myers_verify_args_t *>(args), const_cast<myers_queue_plan_t *>(plan)};
-    return (int)hipLaunchKernel(queue_kernel(plan->cls, args->threshold >= 0.0f), dim3(grid), dim3(64), kargs,
-                                (size_t)plan->ws_u4 * 16u, stream);
+    return (int)hipLaunchKernel(queue_kernel(plan->cls, args->threshold >= 0.0f), dim3(grid),
+                                dim3(64u * myers_queue_waves(plan->cls)), kargs, (size_t)plan->ws_u4 * 16u, stream);
 }
 
 extern "C" int myers_launch_long(const myers_verify_args_t *args, const myers_long_plan_t *plan, uint32_t grid,
@@ -1926,7 +1977,9 @@ extern "C" int myers_scratch_init(int device, uint32_t **ptr) {
 }
 
 extern "C" int myers_variant_serve_ok(uint32_t v, uint32_t *ws_u4) {
-    return myers_variant_queue_class(v, ws_u4) >= 0 ? 1 : 0;
+    if (myers_variant_queue_class(v, nullptr) < 0) return 0;
+    if (ws_u4) *ws_u4 = kWorkspaceU4[v];                // one-wave workgroups: no hand-over words
+    return 1;
 }
 
 extern "C" int myers_serve_occupancy(int cut, uint32_t lds_bytes, int *wg_per_cu) {

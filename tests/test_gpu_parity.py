@@ -520,6 +520,42 @@ def test_queue_launch_every_variant():
         np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b, 0.1, O.MODE_ALIGNED))
 
 
+def test_queue_launch_shared_staging():
+    """The multi-lane queue kernel's two-wave workgroups share one staged Peq:
+    every multi-lane queue variant with one candidate per read (more distinct
+    reads in a chunk pair than query slots: several passes), with three per
+    read (reads straddling the two waves), and with chunk counts that leave
+    the pair's second wave empty, in both window modes, against the oracle."""
+    ref_size = 1_000_000
+    ref = M.synth_reference(43, ref_size)
+    with M.Myers(ref, ref_size, avg_query=4000, cands_per_query=3, buffer_bytes=256 << 20) as m:
+        variants = m.variants()
+        default = m.bin_table()
+        multi = [v for v, info in enumerate(variants) if 2 <= info["lanes"] <= 64 and info["words"] <= 12]
+        assert len(multi) >= 15
+        flags = M.ENGINE_PERSISTENT | M.NO_LONG_LAUNCH
+        for v in multi:
+            info = variants[v]
+            cap = info["maxQuerySize"]
+            slots = 64 // info["lanes"]
+            table = default.copy()
+            w = np.arange(len(table))
+            table[(w >= 1) & (w * 32 <= cap)] = v
+            m.set_bin_table(table)
+            lengths = sorted({max(1, cap - 32 * info["words"] * info["lanes"] + 1), cap - 9, cap})
+            for k, (per_read, reads) in enumerate([(1, 2 * slots + 1), (3, (2 * slots) // 3 + 2)]):
+                announce(f"shared staging variant {v} lanes={info['lanes']} words={info['words']} "
+                         f"candidates/read={per_read} reads={reads}")
+                b = M.synth_batch(M.make_spec(700 + 4 * v + k, reads, choices=lengths, sub=0.04, ins=0.01,
+                                              dele=0.01, candidates=per_read, distance=0.1), ref, ref_size)
+                mode = M.WINDOW_EXACT if (v + k) % 2 else M.WINDOW_ALIGNED
+                got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=mode | flags)
+                assert m.stats().launches == 1
+                np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b, 0.1, mode),
+                                              err_msg=f"variant {info} candidates/read {per_read}")
+        m.set_bin_table(default)
+
+
 def test_reference_beyond_32_bit_positions():
     """A 4.5 Gbase reference (loaded from memory with 64-bit sizes): random
     loci over the whole reference, candidates placed around 2^31 and 2^32,

@@ -78,7 +78,7 @@ CONFIGS = {
 DISTANCE = 0.1
 # HBM bytes per launch of the dominant kernel, from FETCH_SIZE and WRITE_SIZE
 # counter runs scaled by a 1 GiB copy calibration (profiles/hbm_traffic_*.md).
-MEASURED_TRAFFIC = {(2, "aligned"): 1423.9e6, (3, "aligned"): 9.369e9}
+MEASURED_TRAFFIC = {(2, "aligned"): 1423.9e6, (3, "aligned"): 9.547e9}
 
 
 def log(msg):

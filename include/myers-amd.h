@@ -45,7 +45,9 @@ typedef enum {
                                       from that word (src-api/coreMyersGPU-CC35.cu:132,144,164) */
 #define MYERS_WINDOW_EXACT     1u  /* exact start: n columns from the position
                                       (src/myers-4bits.c:100-147) */
-/* Engine selection (extension). */
+/* Engine selection (extension).  AUTO: the plain engine for a batch of one
+ * length bin of up to 2048 candidates per CU, the persistent engine
+ * otherwise. */
 #define MYERS_ENGINE_AUTO       0u
 #define MYERS_ENGINE_PLAIN      (1u << 4)  /* one launch per length bin */
 #define MYERS_ENGINE_PERSISTENT (2u << 4)  /* one persistent launch over a length-binned device queue
@@ -266,6 +268,7 @@ typedef struct {
     uint32_t residentWorkgroups;   /* one-wave workgroups per launch */
     uint32_t residentLdsBytes;     /* LDS per workgroup */
     uint64_t residentItems;        /* work items published */
+    uint64_t liveRequests;         /* submitted and not yet released by a wait */
 } myersServeStats_t;
 
 void myersServeDefaultConfig(myersServeConfig_t *cfg);
@@ -275,7 +278,10 @@ int  myersServeStart(void **server, void **myersBuffers, uint32_t numBuffers, co
 /* Queue a request; the input arrays must stay valid until the request is
  * admitted (myersServeWait returns), results until it completes. */
 int  myersServeSubmit(void *server, const myersRequest_t *req, uint64_t *ticket);
-/* Block until the request completes; latencyMs gets submit-to-completion time. */
+/* Block until the request completes; latencyMs gets submit-to-completion
+ * time.  The wait releases the ticket: the server frees the request, and a
+ * later wait on the same ticket returns MYERS_E_INVALID_ARGUMENT (waits that
+ * are already blocked on it all return normally). */
 int  myersServeWait(void *server, uint64_t ticket, double *latencyMs);
 int  myersServeGetStats(void *server, myersServeStats_t *out);
 /* Drain every queued request, stop the server thread, release the buffers. */

@@ -64,6 +64,9 @@ typedef struct req {
     uint32_t       items;                     /* in both rings */
     resEntry_t    *res;                       /* results in the arena */
     int            too_long;
+    int            in_ring;                   /* held in the resident in-flight ring */
+    int            waiters;                   /* threads inside a wait on it */
+    int            released;                  /* waited on: the caller is done with the ticket */
 } req_t;
 
 /* Resident mode state (MYERS_SERVE_RESIDENT). */
@@ -118,8 +121,13 @@ typedef struct {
     slot_t            *slots;
     uint32_t           nslots;
     req_t             *head, *tail;
+    /* Requests by ticket: table[t - table_base] for table_base <= t <
+     * next_ticket, NULL once freed.  A request is freed when its waiter has
+     * released it and the server thread no longer refers to it; the table
+     * drops its freed prefix when it fills, so it spans the tickets between
+     * the oldest unreleased request and the newest. */
     req_t            **table;
-    uint64_t           table_cap, next_ticket;
+    uint64_t           table_cap, table_base, next_ticket;
     uint64_t           inflight_bytes;
     int                stopping;
     int                thread_started;
@@ -132,6 +140,26 @@ static uint64_t now_ns(void)
     struct timespec ts;
     clock_gettime(CLOCK_MONOTONIC, &ts);
     return (uint64_t)ts.tv_sec * 1000000000ull + (uint64_t)ts.tv_nsec;
+}
+
+/* The request of a ticket, NULL if it was never issued or is already
+ * released.  Caller holds the lock. */
+static req_t *req_of(server_t *sv, uint64_t ticket)
+{
+    if (ticket < sv->table_base || ticket >= sv->next_ticket) return NULL;
+    req_t *q = sv->table[ticket - sv->table_base];
+    return (q != NULL && !q->released) ? q : NULL;
+}
+
+/* Free a request once a wait released it, no other wait is still inside,
+ * and the server thread holds no reference (it is done and out of the
+ * resident ring).  Caller holds the lock. */
+static void req_try_free(server_t *sv, req_t *q)
+{
+    if (!q->released || q->waiters != 0 || q->state != REQ_DONE || q->in_ring) return;
+    sv->table[q->ticket - sv->table_base] = NULL;
+    sv->stats.liveRequests--;
+    free(q);
 }
 
 /* Device bytes a request occupies while admitted: Peq, qinfo, candidates,
@@ -483,6 +511,7 @@ static int res_admit(server_t *sv, req_t *q)
     q->too_long = too_long != 0;
     q->t_admit_ns = t;
     q->state = REQ_ADMITTED;
+    q->in_ring = 1;
     rs->inflight[slot] = q;
     rs->admitted++;
     sv->inflight_bytes += q->bytes;
@@ -523,6 +552,10 @@ static int res_complete(server_t *sv)
         rs->oldest++;
         rs->arena_tail = rs->oldest == rs->admitted ? rs->arena_head
                                                     : rs->inflight[rs->oldest % RES_REQ_SLOTS]->arena_start;
+        if (q != NULL) {
+            q->in_ring = 0;
+            req_try_free(sv, q);
+        }
     }
     if (progressed) {
         sv->stats.completeMs += (double)(now_ns() - t0) / 1e6;
@@ -729,19 +762,29 @@ int myersServeSubmit(void *server, const myersRequest_t *req, uint64_t *ticket)
         free(q);
         return MYERS_E_INVALID_ARGUMENT;
     }
-    if (sv->next_ticket == sv->table_cap) {
-        uint64_t cap = sv->table_cap ? 2u * sv->table_cap : 1024u;
-        req_t **t = (req_t **)realloc(sv->table, cap * sizeof(req_t *));
-        if (t == NULL) {
-            pthread_mutex_unlock(&sv->mu);
-            free(q);
-            return MYERS_E_ALLOCATE_MEM;
+    if (sv->next_ticket - sv->table_base == sv->table_cap) {
+        /* drop the freed prefix; grow only when the live span fills the table */
+        uint64_t k = 0;
+        while (k < sv->table_cap && sv->table[k] == NULL) k++;
+        if (k > 0) {
+            memmove(sv->table, sv->table + k, (size_t)(sv->table_cap - k) * sizeof(req_t *));
+            sv->table_base += k;
         }
-        sv->table = t;
-        sv->table_cap = cap;
+        if (sv->next_ticket - sv->table_base == sv->table_cap) {
+            uint64_t cap = sv->table_cap ? 2u * sv->table_cap : 1024u;
+            req_t **t = (req_t **)realloc(sv->table, cap * sizeof(req_t *));
+            if (t == NULL) {
+                pthread_mutex_unlock(&sv->mu);
+                free(q);
+                return MYERS_E_ALLOCATE_MEM;
+            }
+            sv->table = t;
+            sv->table_cap = cap;
+        }
     }
     q->ticket = sv->next_ticket++;
-    sv->table[q->ticket] = q;
+    sv->table[q->ticket - sv->table_base] = q;
+    sv->stats.liveRequests++;
     if (sv->tail) sv->tail->next = q;
     else sv->head = q;
     sv->tail = q;
@@ -751,15 +794,22 @@ int myersServeSubmit(void *server, const myersRequest_t *req, uint64_t *ticket)
     return MYERS_SUCCESS;
 }
 
+/* Wait for a ticket and release it (a ticket is waited on once).  Caller
+ * holds the lock. */
 static int wait_locked(server_t *sv, uint64_t ticket, uint64_t *t_submit, uint64_t *t_admit, uint64_t *t_done)
 {
-    if (ticket >= sv->next_ticket) return MYERS_E_INVALID_ARGUMENT;
-    req_t *q = sv->table[ticket];
+    req_t *q = req_of(sv, ticket);
+    if (q == NULL) return MYERS_E_INVALID_ARGUMENT;
+    q->waiters++;
     while (q->state != REQ_DONE) pthread_cond_wait(&sv->cv_done, &sv->mu);
+    q->waiters--;
     if (t_submit) *t_submit = q->t_submit_ns;
     if (t_admit) *t_admit = q->t_admit_ns;
     if (t_done) *t_done = q->t_done_ns;
-    return q->error;
+    const int error = q->error;
+    q->released = 1;                      /* a later wait finds the ticket gone */
+    req_try_free(sv, q);
+    return error;
 }
 
 int myersServeWait(void *server, uint64_t ticket, double *latencyMs)
@@ -797,7 +847,7 @@ int myersServeStop(void *server)
     pthread_mutex_unlock(&sv->mu);
     if (sv->thread_started) pthread_join(sv->thread, NULL);
     res_free(sv->res);
-    for (uint64_t i = 0; i < sv->next_ticket; i++) free(sv->table[i]);
+    for (uint64_t i = sv->table_base; i < sv->next_ticket; i++) free(sv->table[i - sv->table_base]);
     free(sv->table);
     for (uint32_t i = 0; i < sv->nslots; i++) {
         free(sv->slots[i].reqs);

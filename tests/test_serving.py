@@ -214,3 +214,32 @@ def test_resident_variant_boundaries():
                 sv.wait(sv.submit(r))
             want = O.myers_batch(ref, REF_SIZE, req[0], req[1], req[2], 0.1, mode)
             np.testing.assert_array_equal(keep[3], want)
+
+
+@pytest.mark.parametrize("serve", MODES, ids=MODE_IDS)
+def test_server_releases_requests(serve):
+    """A wait releases its ticket: after 1,600 small requests (submitted and
+    waited in rounds of 16) the server holds none, and waiting again on a
+    released ticket is an error.  Results stay equal to the oracle."""
+    announce(f"serve release flags={serve}")
+    ref, reqs = make_pool(48)
+    small = sorted(reqs, key=lambda r: len(r[2]))[:8]
+    want = [oracle_req(ref, r) for r in small]
+    with M.Myers(ref, REF_SIZE, num_buffers=2, avg_query=800, cands_per_query=11, buffer_bytes=64 << 20) as m:
+        with M.Server(m, distance=0.1, flags=serve) as sv:
+            peak = 0
+            for rnd in range(100):
+                made = [M.Server.request(*small[i % len(small)]) for i in range(16)]
+                tickets = [sv.submit(x[0]) for x in made]
+                peak = max(peak, sv.stats().liveRequests)
+                for t in tickets:
+                    sv.wait(t)
+                if rnd % 25 == 0:
+                    for i, (req, keep) in enumerate(made):
+                        np.testing.assert_array_equal(keep[3], want[i % len(small)])
+            st = sv.stats()
+            assert st.requests == 1600
+            assert st.liveRequests == 0
+            assert peak <= 16
+            with pytest.raises(M.MyersError):
+                sv.wait(tickets[-1])

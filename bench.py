@@ -76,9 +76,11 @@ CONFIGS = {
         log_uniform=(2000, 20000), err_range=(0.12, 0.12), sub_frac=0.5, candidates=4)),
 }
 DISTANCE = 0.1
-# HBM bytes per launch of the dominant kernel, from FETCH_SIZE and WRITE_SIZE
-# counter runs scaled by a 1 GiB copy calibration (profiles/hbm_traffic_*.md).
-MEASURED_TRAFFIC = {(2, "aligned"): 1423.9e6, (3, "aligned"): 9.547e9}
+# HBM bytes per launch of the dominant kernel under the auto engine (the queue
+# launch for both), from FETCH_SIZE and WRITE_SIZE counter runs scaled by a 1 GiB
+# copy calibration (profiles/verify_queue_kernel_config2.md,
+# profiles/hbm_traffic_verify_queue_config3.md).
+MEASURED_TRAFFIC = {(2, "aligned"): 1.624e9, (3, "aligned"): 9.547e9}
 
 
 def log(msg):

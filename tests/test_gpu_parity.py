@@ -71,9 +71,9 @@ def test_golden_fixtures(path):
 
 
 @pytest.mark.parametrize("mode", [M.WINDOW_ALIGNED, M.WINDOW_EXACT])
-def test_config1_full(mode):
+def test_short_reads_full(mode):
     """4 Mbase seed-42 reference, 10K x 100 bp, 11 candidates each (110K)."""
-    announce(f"config1 mode={mode}")
+    announce(f"10K x 100-base reads mode={mode}")
     ref, b = synth_case(42, 4_000_000, 10_000, length=100, sub=0.02, ins=0.0025, dele=0.0025,
                         candidates=11, distance=0.1)
     with M.Myers(ref, 4_000_000, avg_query=100, cands_per_query=11, buffer_bytes=256 << 20) as m:
@@ -84,7 +84,7 @@ def test_config1_full(mode):
 
 
 def test_mixed_lengths_binned():
-    """Config-3 style mix through device-side binning (several variants in one batch)."""
+    """Reads of 100-1000 bases mixed in one batch through device-side binning (several variants)."""
     announce("mixed lengths")
     ref, b = synth_case(5, 2_000_000, 600, choices=[100, 200, 400, 600, 800, 1000], err_range=(0.01, 0.05),
                         sub_frac=0.8, poisson=11, cap=64, distance=0.1)
@@ -97,7 +97,7 @@ def test_mixed_lengths_binned():
 
 @pytest.mark.parametrize("mode", [M.WINDOW_ALIGNED, M.WINDOW_EXACT])
 def test_long_reads(mode):
-    """Config-4 style: 2-20 kb reads, 12% error, over the multi-lane variants."""
+    """Long noisy reads: 2-20 kb, 12% error, over the multi-lane variants."""
     announce(f"long reads mode={mode}")
     ref, b = synth_case(11, 3_000_000, 40, log_uniform=(2000, 20000), err_range=(0.12, 0.12), sub_frac=0.5,
                         candidates=4, distance=0.1)
@@ -461,10 +461,10 @@ def test_long_read_launch_edge_cases():
 
 
 @pytest.mark.parametrize("mode", [M.WINDOW_ALIGNED, M.WINDOW_EXACT])
-def test_queue_launch_config3_mix(mode):
-    """Config-3 mix (six one-lane bins) through the persistent engine: one
+def test_queue_launch_mixed_short_reads(mode):
+    """Reads of 100-1000 bases (six one-lane bins) through the persistent engine: one
     queue launch over every bin, bit-exact against the oracle."""
-    announce(f"queue launch config-3 mix mode={mode}")
+    announce(f"queue launch mixed 100-1000-base reads mode={mode}")
     ref, b = synth_case(41, 3_000_000, 1500, choices=[100, 200, 400, 600, 800, 1000], err_range=(0.01, 0.05),
                         sub_frac=0.8, poisson=11, cap=64, distance=0.1)
     with M.Myers(ref, 3_000_000, avg_query=550, cands_per_query=11, buffer_bytes=256 << 20) as m:

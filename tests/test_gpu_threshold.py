@@ -58,7 +58,7 @@ def test_threshold_every_single_lane_variant():
 
 
 @pytest.mark.parametrize("mode", [M.WINDOW_ALIGNED, M.WINDOW_EXACT])
-def test_threshold_config2_shape(mode):
+def test_threshold_1000_base_reads(mode):
     """1000-base reads, 11 candidates each (one true locus), e = 0.2, both engines."""
     size = 3_000_000
     ref = M.synth_reference(62, size)
@@ -71,7 +71,7 @@ def test_threshold_config2_shape(mode):
         m.set_threshold(0.2)
         assert abs(m.threshold() - 0.2) < 1e-7
         for engine in (M.ENGINE_PLAIN, M.ENGINE_PERSISTENT):
-            announce(f"threshold config-2 shape mode={mode} engine={engine}")
+            announce(f"threshold 1000-base reads mode={mode} engine={engine}")
             got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1, flags=mode | engine)
             np.testing.assert_array_equal(got, want)
         m.set_threshold(-1.0)

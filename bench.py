@@ -354,7 +354,7 @@ def measure_serving(m, ref, ref_size, loads, seconds, batch_cells=0, engine=0, s
            "mean_candidates_per_request": float(np.mean(cands)),
            "server": "resident kernel" if resident else "continuous batching",
            "buffers": 1 if resident else m.num_buffers,
-           "max_batch_cells": None if resident else (batch_cells or "library default (2.5e11)"),
+           "max_batch_cells": None if resident else (batch_cells or "library default (5e11)"),
            "engine": engine, "levels": []}
     flags = engine | (M.SERVE_RESIDENT if resident else 0)
     with M.Server(m, distance=DISTANCE, flags=flags, max_batch_cells=batch_cells) as sv:

@@ -237,7 +237,7 @@ typedef struct {
     uint32_t maxBatchCandidates;/* cap per batch (0 = slot capacity) */
     uint64_t maxBatchCells;     /* cap per batch on the sum of m * n over its candidates, so
                                    that batches stay short and several run at once
-                                   (0 = 2.5e11; UINT64_MAX = no cap); a request larger than
+                                   (0 = 5e11; UINT64_MAX = no cap); a request larger than
                                    the cap runs as a batch of its own */
     uint64_t arenaBytes;        /* MYERS_SERVE_RESIDENT: pinned host memory for admitted
                                    requests' inputs and results (0 = 1 GiB) */

@@ -162,11 +162,11 @@ static void req_try_free(server_t *sv, req_t *q)
     free(q);
 }
 
-/* Device bytes a request occupies while admitted: Peq, qinfo, candidates,
- * results and the binning slot of each candidate. */
-/* About 2.5 ms of verification at 100 TCUPS: short enough that several
- * batches are in flight and a request does not wait behind a long one. */
-#define DEFAULT_BATCH_CELLS 250000000000ull
+/* About 5 ms of verification at 100 TCUPS.  Batches carry enough long
+ * candidates (one chain per wave each) to keep the chip busy with three in
+ * flight: on the serving mix at 0.7 of the bulk rate, p99 19 ms against 28
+ * ms with 2.5e11-cell batches (3 buffers, same box, two runs each). */
+#define DEFAULT_BATCH_CELLS 500000000000ull
 
 /* Sum of m * n over the request's candidates (nominal windows). */
 static uint64_t request_cells(const myersRequest_t *r, float distance)
@@ -181,6 +181,8 @@ static uint64_t request_cells(const myersRequest_t *r, float distance)
     return cells;
 }
 
+/* Device bytes a request occupies while admitted: Peq, qinfo, candidates,
+ * results and the binning slot of each candidate. */
 static uint64_t request_bytes(const myersRequest_t *r)
 {
     return (uint64_t)r->numPEQEntries * sizeof(qryEntry_t) + (uint64_t)r->numQueries * sizeof(qryInfo_t) +

@@ -333,7 +333,8 @@ def concat_requests(reqs):
     return np.concatenate(peqs), np.concatenate(qis), np.concatenate(cands)
 
 
-def measure_serving(m, ref, ref_size, loads, seconds, batch_cells=0, engine=0, seed=5, resident=True):
+def measure_serving(m, ref, ref_size, loads, seconds, batch_cells=0, engine=0, seed=5, resident=True,
+                    no_long_launch=False):
     pool = serving_pool(ref, ref_size, seed, 192)
     cells = [int((r[1][r[2]["query"], 1].astype(np.int64) * window_len(r[1][r[2]["query"], 1])).sum()) for r in pool]
     cands = [len(r[2]) for r in pool]
@@ -356,7 +357,9 @@ def measure_serving(m, ref, ref_size, loads, seconds, batch_cells=0, engine=0, s
            "buffers": 1 if resident else m.num_buffers,
            "max_batch_cells": None if resident else (batch_cells or "library default (5e11)"),
            "engine": engine, "levels": []}
-    flags = engine | (M.SERVE_RESIDENT if resident else 0)
+    flags = engine | (M.SERVE_RESIDENT if resident else 0) | (M.NO_LONG_LAUNCH if no_long_launch and not resident else 0)
+    if no_long_launch and not resident:
+        out["no_long_launch"] = True
     with M.Server(m, distance=DISTANCE, flags=flags, max_batch_cells=batch_cells) as sv:
         # saturated throughput: every pool request arrives at once
         made = [M.Server.request(*r) for r in pool]
@@ -567,6 +570,8 @@ def main():
     ap.add_argument("--serve-buffers", type=int, default=3)
     ap.add_argument("--serve-batch-cells", type=float, default=0, help="serving batch cap on cells (0 = library default)")
     ap.add_argument("--serve-engine", default="auto", choices=["auto", "plain", "persistent", "binned"])
+    ap.add_argument("--serve-no-long-launch", action="store_true",
+                    help="serving batches keep long reads in the queue launch (no long-read launch)")
     ap.add_argument("--serve-mode", default="both", choices=["both", "resident", "batches"],
                     help="serving by continuous batching over the buffers, by one resident kernel, or both "
                          "(batching is the reported 'serving' line, the resident kernel 'serving_resident')")
@@ -618,7 +623,8 @@ def main():
     if serve_loads and args.serve_mode in ("both", "batches"):
         log("serving (continuous batching) ...")
         serving = measure_serving(m, ref, cfg["ref_size"], serve_loads, args.serve_seconds,
-                                  int(args.serve_batch_cells), engines[args.serve_engine], resident=False)
+                                  int(args.serve_batch_cells), engines[args.serve_engine], resident=False,
+                                  no_long_launch=args.serve_no_long_launch)
     if serve_loads and args.serve_mode in ("both", "resident"):
         log("serving (resident kernel) ...")
         serving_resident = measure_serving(m, ref, cfg["ref_size"], serve_loads, args.serve_seconds,

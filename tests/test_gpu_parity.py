@@ -263,6 +263,24 @@ def test_persistent_single_length():
     np.testing.assert_array_equal(got, oracle_for(ref, 1_000_000, b, 0.1, 0))
 
 
+def test_auto_engine_by_batch_size():
+    """The auto engine runs a one-length batch on the plain engine (no
+    binning) up to 2048 candidates per CU and on the queue launch (binned)
+    above that (256 CUs: 524,288 candidates); both equal the oracle."""
+    ref_size = 2_000_000
+    ref = M.synth_reference(33, ref_size)
+    announce("auto engine by batch size")
+    with M.Myers(ref, ref_size, avg_query=100, cands_per_query=11, buffer_bytes=256 << 20) as m:
+        for reads, binned in ((10_000, 0), (50_000, 1)):
+            b = M.synth_batch(M.make_spec(34 + reads, reads, length=100, sub=0.03, ins=0.005, dele=0.005,
+                                          candidates=11, distance=0.1), ref, ref_size)
+            got = m.verify(b["peq"], b["qinfo"], b["cands"], 0.1)
+            st = m.stats()
+            assert st.binned == binned, f"{len(b['cands'])} candidates"
+            assert st.launches == 1
+            np.testing.assert_array_equal(got, oracle_for(ref, ref_size, b, 0.1, 0))
+
+
 @pytest.mark.parametrize("engine", [M.ENGINE_PLAIN, M.ENGINE_PERSISTENT, M.ENGINE_BINNED])
 def test_graph_replays_match_eager(engine):
     """A captured pipeline, replayed several times with eager work and

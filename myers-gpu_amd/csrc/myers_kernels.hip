@@ -183,93 +183,55 @@ __device__ __forceinline__ uint32_t from_lower_lane(uint32_t v, uint32_t not_fir
     return v;
 }
 
-// One group of four words (g) of one column: the recurrence over words
-// 4g..4g+3 with the Eq words e (src/myers-4bits.c:115-134 block form).  carry
-// is the add's carry between words; ph_prev / mh_prev the Ph / Mh (before the
-// shift) of the word below, whose top bit shifts into this group's first word.
-//
-// kPhased: issue the group's word updates in phases (below).  With several
-// waves per SIMD this raises the share of quad-cycles that issue two VALU
-// instructions (18% -> 35% on 1000-base reads, 11% faster); a wave that runs
-// a long chain alone on its SIMD (the long-read launch, multi-wave
-// candidates) is 5-10% faster with the compiler's interleaved order, which
-// overlaps the carry chain with the other words' operations.
-template <int W, bool kHin, bool kPhased>
+// One group of four words (g) of a one-lane column (L = 1), for the cut-off
+// column below: the recurrence over words 4g..4g+3 with the Eq words e
+// (src/myers-4bits.c:115-134 block form), in the phased order of column_grp.
+// carry is the add's carry between words; ph_prev / mh_prev the Ph / Mh
+// (before the shift) of the word below, whose top bit shifts into this
+// group's first word.
+template <int W>
 __device__ __forceinline__ void group_update(int g, uint32_t (&Pv)[W], uint32_t (&Mv)[W], const uint4 &e,
-                                             uint32_t mh_in, uint32_t &carry, uint32_t &ph_prev,
-                                             uint32_t &mh_prev) {
+                                             uint32_t &carry, uint32_t &ph_prev, uint32_t &mh_prev) {
     const uint32_t e4[4] = {e.x, e.y, e.z, e.w};
-    if constexpr (kPhased) {
-        const int k0 = 4 * g;
-        const int n = (W - k0) < 4 ? (W - k0) : 4;
-        uint32_t eqv[4], t[4], xv[4], sum[4], xh[4], ph[4], mh[4], php[4], mhp[4];
-        // The group's words go through the recurrence in phases, each
-        // phase over all of its words, so that the double-rate bit
-        // operations of independent words issue back to back (a SIMD
-        // issues two of them in one quad-cycle) and the single-rate ones
-        // (add with carry, funnel shifts) come in runs:
-        //   Eq & Pv, Xv | carry chain | Xh, Ph, Mh | shifts | Pv, Mv.
-        // sched_barrier keeps the compiler from interleaving the phases.
+    const int k0 = 4 * g;
+    const int n = (W - k0) < 4 ? (W - k0) : 4;
+    uint32_t t[4], xv[4], sum[4], xh[4], ph[4], mh[4], php[4], mhp[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            if (j >= n) break;
-            const int k = k0 + j;
-            eqv[j] = (kHin && k == 0) ? (e4[0] | (mh_in >> 31)) : e4[j];
-            t[j] = eqv[j] & Pv[k];
-            xv[j] = e4[j] | Mv[k];
-        }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            if (j >= n) break;
-            uint32_t c_out;
-            sum[j] = __builtin_addc(t[j], Pv[k0 + j], carry, &c_out);
-            carry = c_out;
-        }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int j = 0; j < 4 && j < n; j++) xh[j] = __builtin_amdgcn_bitop3_b32(sum[j], Pv[k0 + j], eqv[j], kXorOr);
-#pragma unroll
-        for (int j = 0; j < 4 && j < n; j++) {
-            ph[j] = __builtin_amdgcn_bitop3_b32(Mv[k0 + j], xh[j], Pv[k0 + j], kOrNor);
-            mh[j] = Pv[k0 + j] & xh[j];
-        }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int j = 0; j < 4 && j < n; j++) {
-            const int k = k0 + j;
-            php[j] = (!kHin && k == 0) ? (ph[j] << 1) : __builtin_amdgcn_alignbit(ph[j], ph_prev, 31u);
-            mhp[j] = (!kHin && k == 0) ? (mh[j] << 1) : __builtin_amdgcn_alignbit(mh[j], mh_prev, 31u);
-            ph_prev = ph[j];
-            mh_prev = mh[j];
-        }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int j = 0; j < 4 && j < n; j++) {
-            Pv[k0 + j] = __builtin_amdgcn_bitop3_b32(mhp[j], xv[j], php[j], kOrNor);
-            Mv[k0 + j] = php[j] & xv[j];
-        }
-        __builtin_amdgcn_sched_barrier(0);
-    } else {
-#pragma unroll
-        for (int k = 4 * g; k < 4 * g + 4 && k < W; k++) {
-            const uint32_t pv = Pv[k], mv = Mv[k];
-            const uint32_t xv = e4[k - 4 * g] | mv;
-            const uint32_t eq = (kHin && k == 0) ? (e4[0] | (mh_in >> 31)) : e4[k - 4 * g];
-            uint32_t c_out;
-            const uint32_t sum = __builtin_addc(eq & pv, pv, carry, &c_out);
-            carry = c_out;
-            const uint32_t xh = __builtin_amdgcn_bitop3_b32(sum, pv, eq, kXorOr);
-            const uint32_t ph = __builtin_amdgcn_bitop3_b32(mv, xh, pv, kOrNor);
-            const uint32_t mh = pv & xh;
-            const uint32_t php = (!kHin && k == 0) ? (ph << 1) : __builtin_amdgcn_alignbit(ph, ph_prev, 31u);
-            const uint32_t mhp = (!kHin && k == 0) ? (mh << 1) : __builtin_amdgcn_alignbit(mh, mh_prev, 31u);
-            ph_prev = ph;
-            mh_prev = mh;
-            Pv[k] = __builtin_amdgcn_bitop3_b32(mhp, xv, php, kOrNor);
-            Mv[k] = php & xv;
-        }
+    for (int j = 0; j < 4 && j < n; j++) {
+        t[j] = e4[j] & Pv[k0 + j];
+        xv[j] = e4[j] | Mv[k0 + j];
     }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < 4 && j < n; j++) {
+        uint32_t c_out;
+        sum[j] = __builtin_addc(t[j], Pv[k0 + j], carry, &c_out);
+        carry = c_out;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < 4 && j < n; j++) xh[j] = __builtin_amdgcn_bitop3_b32(sum[j], Pv[k0 + j], e4[j], kXorOr);
+#pragma unroll
+    for (int j = 0; j < 4 && j < n; j++) {
+        ph[j] = __builtin_amdgcn_bitop3_b32(Mv[k0 + j], xh[j], Pv[k0 + j], kOrNor);
+        mh[j] = Pv[k0 + j] & xh[j];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < 4 && j < n; j++) {
+        const int k = k0 + j;
+        php[j] = k == 0 ? (ph[j] << 1) : __builtin_amdgcn_alignbit(ph[j], ph_prev, 31u);
+        mhp[j] = k == 0 ? (mh[j] << 1) : __builtin_amdgcn_alignbit(mh[j], mh_prev, 31u);
+        ph_prev = ph[j];
+        mh_prev = mh[j];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < 4 && j < n; j++) {
+        Pv[k0 + j] = __builtin_amdgcn_bitop3_b32(mhp[j], xv[j], php[j], kOrNor);
+        Mv[k0 + j] = php[j] & xv[j];
+    }
+    __builtin_amdgcn_sched_barrier(0);
 }
 
 // One column for one lane's W words.  Eq words stream from LDS one group of
@@ -279,10 +241,15 @@ __device__ __forceinline__ void group_update(int g, uint32_t (&Pv)[W], uint32_t 
 // lower lane's hin in bit 31 (src/myers-4bits.c:115-134 block form).  The
 // top word's Ph / Mh (before the shift) come back in ph_top / mh_top.
 //
-// kPhased: as in group_update.  The body repeats group_update's rather than
-// calling it: through the call the compiler sinks Xv into the last phase,
-// which measured 2% slower here, while pinning Xv in the first phase makes
-// the cut-off column (which calls group_update) spill.
+// kPhased: issue each group's word updates in phases (below).  With several
+// waves per SIMD this raises the share of quad-cycles that issue two VALU
+// instructions (18% -> 35% on 1000-base reads, 11% faster); a wave that runs
+// a long chain alone on its SIMD (the long-read launch, multi-wave
+// candidates) is 5-10% faster with the compiler's interleaved order, which
+// overlaps the carry chain with the other words' operations.  The phased
+// body is also group_update's; calling group_update here measured 2% slower
+// (the compiler sinks Xv into the last phase), and pinning Xv in the first
+// phase makes the cut-off column spill.
 template <int W, bool kHin, uint32_t kGS, bool kPhased = true>
 __device__ __forceinline__ void column_grp(uint32_t (&Pv)[W], uint32_t (&Mv)[W], const uint4 &e0,
                                            const uint4 *lds_lane, uint32_t cur_off, uint32_t ph_in,
@@ -388,7 +355,7 @@ __device__ __forceinline__ void column_cut(uint32_t (&Pv)[W], uint32_t (&Mv)[W],
     constexpr int kG = (W + 3) / 4;
     uint4 nxt = cur;
     if constexpr (G + 1 < kG) nxt = lds_lane[cur_off + (uint32_t)(G + 1) * kGS];
-    group_update<W, false, true>(G, Pv, Mv, cur, 0u, carry, ph_prev, mh_prev);
+    group_update<W>(G, Pv, Mv, cur, carry, ph_prev, mh_prev);
     if constexpr (G + 1 < kG) {
         if ((uint32_t)(G + 1) < ga) {
             column_cut<G + 1, W, kGS>(Pv, Mv, nxt, lds_lane, cur_off, ga, carry, ph_prev, mh_prev, ph_top, mh_top);

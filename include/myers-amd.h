@@ -129,8 +129,11 @@ int myersReceiveEx(void *myersBuffer);
  * {0, 0xFFFFFFFF}.  Kernels for queries of up to 1024 bases then skip the
  * rows whose distance already exceeds the threshold (Ukkonen cut-off), and
  * the multi-lane kernels of per-bin launches stop a wave once none of its
- * candidates can get under the threshold.  errorRate < 0 switches the mode
- * off (the default). */
+ * candidates can get under the threshold.  In binned batches a probe of each
+ * such query's first 32 bases orders those candidates so that the ones
+ * likely under the threshold share waves (the order of the work only; the
+ * results stay in the caller's order).  errorRate < 0 switches the mode off
+ * (the default). */
 #define MYERS_NO_MATCH 0xFFFFFFFFu
 int   myersSetThreshold(void *myersBuffer, float errorRate);
 float myersGetThreshold(void *myersBuffer);

@@ -64,6 +64,7 @@ typedef struct myers_ctx {
     uint8_t  *d_bin_of_words;
     uint64_t  tune_gen;          /* bumped when the bin table or chunk factors change */
     uint32_t *d_scratch;         /* this device's static workspace (myersVerifyDeviceNoAlloc) */
+    uint64_t  probe_mask;        /* threshold mode: variants whose candidates the band probe orders */
 } myers_ctx_t;
 
 typedef struct myers_buffer {
@@ -363,6 +364,7 @@ static int create_context(myers_ctx_t **out, int device)
         return MYERS_E_HIP;
     }
     ctx->num_variants = myers_num_variants();
+    ctx->probe_mask = myers_probe_variants();
     for (uint32_t v = 0; v < ctx->num_variants; v++) {
         myers_variant_desc(v, &ctx->variants[v]);
         int occ = 0;
@@ -821,6 +823,13 @@ static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan
         b->stats.variantsUsed = 1;
         return MYERS_SUCCESS;
     }
+    /* Threshold mode: the band probe classes the candidates of one-lane
+     * cut-off variants (before the binning) and orders their bins (after). */
+    const uint64_t probe = bt->threshold >= 0.0f ? ctx->probe_mask & scan->present : 0u;
+    if (probe) {
+        rc = myers_launch_probe(&a, bt->num_cands, ctx->d_bin_of_words, max_supported_words(ctx), probe, b->stream);
+        if (rc) return hip_fail((hipError_t)rc, "probe launch", __FILE__, __LINE__);
+    }
     /* device-side binning */
     uint32_t *ranges = bt->d_work + MYERS_WORK_RANGES;
     uint32_t *heads = bt->d_work + MYERS_WORK_HEADS;
@@ -830,6 +839,11 @@ static int enqueue_pipeline(myers_buffer_t *b, const batch_t *bt, const bin_scan
                               bt->d_work + MYERS_WORK_BLOCKS, ranges, bt->d_order, err, bt->d_results, heads,
                               b->stream);
     if (rc) return hip_fail((hipError_t)rc, "binning launch", __FILE__, __LINE__);
+    if (probe) {
+        rc = myers_launch_probe_order(ranges, ctx->num_variants, probe, bt->num_cands, bt->d_order, bt->d_results,
+                                      b->stream);
+        if (rc) return hip_fail((hipError_t)rc, "probe order launch", __FILE__, __LINE__);
+    }
     HIP_TRY(hipMemcpyAsync(b->h_err, err, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
     if (b->mark_slot >= 0) {
         /* eager batches: the verification launches are timed apart from the binning */

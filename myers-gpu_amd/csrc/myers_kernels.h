@@ -268,6 +268,28 @@ int myers_launch_binning(const uint32_t *qinfo, uint32_t num_queries, const void
                          uint32_t *block_counts, uint32_t *ranges, uint32_t *order, uint32_t *err,
                          void *results, uint32_t *queue_heads, hipStream_t stream);
 
+/* Threshold mode's band probe.  The cut-off advances a wave of one-lane
+ * candidates as deep as its deepest lane, and a candidate whose read occurs
+ * in its window keeps most of the column's rows within the threshold; in a
+ * batch where such candidates are spread over every wave, every wave runs
+ * deep.  myers_launch_probe (before the binning) classes every candidate
+ * whose variant v has bit v set in probe_mask: the first 32 bases of the read
+ * (one bit-vector word) run over the leading min(n, 2 d m + 72) columns of
+ * its window, where a read placed by the filter begins, and results[i].x
+ * gets 0 when some column has D <= 8 there, else 1; other candidates get 0.
+ * The args' peq, qinfo, cands, ref, mode, distance and results are used.
+ * myers_launch_probe_order (after the binning) reorders each probed
+ * variant's range in spans of two 64-slot chunks so that the first chunk
+ * takes the span's class-0 candidates (then class 1 in slot order), keeping
+ * a read's candidates in runs.  Both only order the work: the results are
+ * those of the batch without them. */
+int myers_launch_probe(const myers_verify_args_t *args, uint32_t num_cands, const uint8_t *bin_of_words,
+                       uint32_t max_words, uint64_t probe_mask, hipStream_t stream);
+int myers_launch_probe_order(const uint32_t *ranges, uint32_t num_variants, uint64_t probe_mask,
+                             uint32_t num_cands, uint32_t *order, const void *results, hipStream_t stream);
+/* The variants worth probing: one-lane variants with a cut-off kernel. */
+uint64_t myers_probe_variants(void);
+
 #ifdef __cplusplus
 }
 #endif

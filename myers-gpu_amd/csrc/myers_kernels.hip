@@ -1728,6 +1728,133 @@ __global__ __launch_bounds__(256) void bin_scatter_kernel(const uint2 *qinfo, ui
     }
 }
 
+
+// ------------------------------------------------------------- band probe
+// Threshold mode (see myers_launch_probe).  probe_kernel: one thread per
+// candidate, the read's first 32 bases as one bit-vector word (Peq word 0 of
+// codes 0..4; the recurrence of src/myers-4bits.c:115-134), D at row 32
+// tracked over the window's leading columns.  Each thread's five Eq words
+// sit in its own LDS column and are read back by the text code of each
+// step; the window's words are all requested before the first step (random
+// windows: the loads' latency, not their bytes, bounds this kernel).
+constexpr uint32_t kProbeLikely = 8u;      // D <= 8 somewhere on the first 32 bases: class 0
+constexpr uint32_t kProbeSpan = 128u;      // slots reordered together: two one-lane chunks
+constexpr uint32_t kProbeBlocks = 40u;     // at most 320 columns probed
+__global__ __launch_bounds__(256) void probe_kernel(myers_verify_args_t a, uint32_t n,
+                                                    const uint8_t *bin_of_words, uint32_t max_words,
+                                                    uint64_t probe_mask) {
+    __shared__ uint32_t eqs[5][256];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t i = blockIdx.x * 256u + tid;
+    uint32_t cls = 0;
+    uint32_t ncols = 0;
+    uint64_t nib0 = 0;
+    if (i < n) {
+        const CandRec cr = static_cast<const CandRec *>(a.cands)[i];
+        const uint2 *qinfo = reinterpret_cast<const uint2 *>(a.qinfo);
+        if (cr.query < a.num_queries) {
+            const uint2 qi = qinfo[cr.query];
+            const uint32_t m = qi.y, w = (m + 31u) >> 5;
+            const uint32_t v = w <= max_words ? (uint32_t)bin_of_words[w] : 64u;
+            if (v < 64u && ((probe_mask >> v) & 1ull) && m >= 32u &&
+                (uint64_t)qi.x + ((m + 127u) >> 7) <= (uint64_t)a.num_peq_entries) {
+                const uint32_t nw = window_len(m, a.distance);
+                // skip rule: src-api/coreMyersGPU-CC35.cu:142
+                if (cr.position < a.ref_size && a.ref_size - cr.position > (uint64_t)nw) {
+                    nib0 = a.mode == 0 ? (cr.position & ~7ull) : cr.position;
+                    const uint32_t lead = (uint32_t)__fmul_rn(__fmul_rn(2.0f, a.distance), (float)m) + 72u;
+                    ncols = min(min(nw, lead), 8u * kProbeBlocks);
+                    for (uint32_t c = 0; c < 5u; c++) eqs[c][tid] = a.peq[(size_t)qi.x * 20u + 4u * c];
+                }
+            }
+        }
+    }
+    const uint32_t wave_cols = wave_max(ncols);
+    if (wave_cols > 0u) {
+        const uint32_t *ref = a.ref;
+        const int64_t last_word = (int64_t)a.ref_words - 1;
+        const int64_t w0 = (int64_t)(nib0 >> 3);
+        const uint32_t shift = (uint32_t)(nib0 & 7u) * 4u;
+        uint32_t tw[kProbeBlocks + 1];
+#pragma unroll
+        for (uint32_t k = 0; k <= kProbeBlocks; k++) tw[k] = ref[min(w0 + (int64_t)k, last_word)];
+        // keep the loads here, ahead of the steps (the compiler would sink
+        // each one to its block, after the early exits)
+#pragma unroll
+        for (uint32_t k = 0; k <= kProbeBlocks; k++) asm volatile("" ::"v"(tw[k]));
+        uint32_t pv = ~0u, mv = 0u;
+        int32_t score = 32, best = 32;
+#pragma unroll
+        for (uint32_t b = 0; b < kProbeBlocks; b++) {
+            if (8u * b >= wave_cols) break;
+            const uint32_t t = sanitize_text(__builtin_amdgcn_alignbit(tw[b + 1], tw[b], shift));
+            uint32_t eq[8];
+#pragma unroll
+            for (uint32_t j = 0; j < 8u; j++) eq[j] = eqs[(t >> (4u * j)) & 7u][tid];
+#pragma unroll
+            for (uint32_t j = 0; j < 8u; j++) {
+                const uint32_t xv = eq[j] | mv;
+                const uint32_t xh = __builtin_amdgcn_bitop3_b32((eq[j] & pv) + pv, pv, eq[j], kXorOr);
+                const uint32_t ph = __builtin_amdgcn_bitop3_b32(mv, xh, pv, kOrNor);
+                const uint32_t mh = pv & xh;
+                score += (int32_t)(ph >> 31) - (int32_t)(mh >> 31);
+                best = 8u * b + j < ncols ? min(best, score) : best;
+                const uint32_t php = ph << 1, mhp = mh << 1;
+                pv = __builtin_amdgcn_bitop3_b32(mhp, xv, php, kOrNor);
+                mv = php & xv;
+            }
+        }
+        if (ncols > 0u) cls = best <= (int32_t)kProbeLikely ? 0u : 1u;
+    }
+    if (i < n) static_cast<uint2 *>(a.results)[i].x = cls;
+}
+
+// Reorder the slots of each probed variant's range in spans of kProbeSpan
+// (two chunks of 64 one-lane candidates, aligned to the range start): the
+// first chunk takes the span's class-0 candidates and then its first class-1
+// ones, the second chunk the rest, each in slot order.  A read's candidates,
+// adjacent in the range, stay in runs, so the first chunk holds about as
+// many distinct reads as class-0 candidates and its Peq still stages in one
+// pass; the second chunk's waves then keep a shallow band.  One wave per
+// span, spans in grid-stride order.
+__global__ __launch_bounds__(64) void probe_order_kernel(const uint32_t *ranges, uint32_t nv, uint64_t probe_mask,
+                                                         uint32_t *order, const uint2 *results) {
+    const uint32_t lane = threadIdx.x;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (uint32_t v = 0; v < nv && v < 64u; v++) {
+        if (!((probe_mask >> v) & 1ull)) continue;
+        const uint32_t start = ranges[2 * v], count = ranges[2 * v + 1];
+        const uint32_t nspans = (count + kProbeSpan - 1u) / kProbeSpan;
+        for (uint32_t sp = blockIdx.x; sp < nspans; sp += gridDim.x) {
+            const uint32_t s0 = start + sp * kProbeSpan;
+            const uint32_t here = min(kProbeSpan, count - sp * kProbeSpan);
+            if (here <= 64u) continue;                  // one chunk: nothing to move
+            const bool va = lane < here, vb = 64u + lane < here;
+            const uint32_t ca = va ? order[s0 + lane] : 0u;
+            const uint32_t cb = vb ? order[s0 + 64u + lane] : 0u;
+            const uint32_t ka = va ? (results[ca].x & 1u) : 1u;
+            const uint32_t kb = vb ? (results[cb].x & 1u) : 1u;
+            const uint64_t z0a = __ballot(va && ka == 0u), z0b = __ballot(vb && kb == 0u);
+            const uint32_t t = (uint32_t)(__builtin_popcountll(z0a) + __builtin_popcountll(z0b));
+            if (t == 0u || t >= 64u) continue;          // wave-uniform
+            const uint64_t o1a = __ballot(va && ka == 1u), o1b = __ballot(vb && kb == 1u);
+            const uint32_t r1a = (uint32_t)__builtin_popcountll(o1a & below);
+            const uint32_t r1b = (uint32_t)(__builtin_popcountll(o1a) + __builtin_popcountll(o1b & below));
+            const uint32_t fill = 64u - t;
+            const bool fa = va && (ka == 0u || r1a < fill);   // to the first chunk
+            const bool fb = vb && (kb == 0u || r1b < fill);
+            const uint64_t fma = __ballot(fa), fmb = __ballot(fb);
+            const uint64_t sma = __ballot(va && !fa), smb = __ballot(vb && !fb);
+            const uint32_t pa = fa ? (uint32_t)__builtin_popcountll(fma & below)
+                                   : 64u + (uint32_t)__builtin_popcountll(sma & below);
+            const uint32_t pb = fb ? (uint32_t)(__builtin_popcountll(fma) + __builtin_popcountll(fmb & below))
+                                   : 64u + (uint32_t)(__builtin_popcountll(sma) + __builtin_popcountll(smb & below));
+            if (va) order[s0 + pa] = ca;
+            if (vb) order[s0 + pb] = cb;
+        }
+    }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------ variant table
@@ -1970,6 +2097,32 @@ extern "C" int myers_launch_serve(const myers_serve_args_t *args, int cut, uint3
                          : reinterpret_cast<const void *>(&verify_serve_kernel<false>);
     void *kargs[] = {const_cast<myers_serve_args_t *>(args)};
     return (int)hipLaunchKernel(fn, dim3(grid), dim3(64), kargs, (size_t)lds_bytes, stream);
+}
+
+extern "C" uint64_t myers_probe_variants(void) {
+    uint64_t mask = 0;
+    for (uint32_t v = 0; v < kNumVariants && v < 64u; v++)
+        if (kVariants[v].lanes == 1u && kVariants[v].cut_fn != nullptr) mask |= 1ull << v;
+    return mask;
+}
+
+extern "C" int myers_launch_probe(const myers_verify_args_t *args, uint32_t num_cands, const uint8_t *bin_of_words,
+                                  uint32_t max_words, uint64_t probe_mask, hipStream_t stream) {
+    if (args == nullptr || args->results == nullptr || num_cands == 0u) return -1;
+    hipLaunchKernelGGL(probe_kernel, dim3((num_cands + 255u) / 256u), dim3(256), 0, stream, *args, num_cands,
+                       bin_of_words, max_words, probe_mask);
+    return (int)hipGetLastError();
+}
+
+extern "C" int myers_launch_probe_order(const uint32_t *ranges, uint32_t num_variants, uint64_t probe_mask,
+                                        uint32_t num_cands, uint32_t *order, const void *results,
+                                        hipStream_t stream) {
+    if (ranges == nullptr || order == nullptr || results == nullptr || num_variants > MYERS_MAX_VARIANTS) return -1;
+    uint32_t grid = num_cands / kProbeSpan + 1u;
+    if (grid > 4096u) grid = 4096u;
+    hipLaunchKernelGGL(probe_order_kernel, dim3(grid), dim3(64), 0, stream, ranges, num_variants, probe_mask, order,
+                       static_cast<const uint2 *>(results));
+    return (int)hipGetLastError();
 }
 
 extern "C" int myers_launch_binning(const uint32_t *qinfo, uint32_t num_queries, const void *cands,

@@ -176,3 +176,36 @@ def test_threshold_queue_launches(mode):
                            flags=mode | M.ENGINE_PERSISTENT | M.NO_LONG_LAUNCH)
             assert m.stats().launches == 2
             np.testing.assert_array_equal(got, expected(ref, size, b, 0.1, mode, e))
+
+
+@pytest.mark.parametrize("mode", [M.WINDOW_ALIGNED, M.WINDOW_EXACT])
+def test_threshold_probe_order(mode):
+    """The band probe and the reordering of probed bins only change the
+    order of the work.  Batches whose 128-slot spans hold no class-0
+    candidate (random windows only), a few (one true locus per read,
+    Poisson candidate counts, so spans cut reads apart), or more than 64
+    (every candidate at its true locus), lengths over one-lane cut-off
+    variants with even and odd W, skipped candidates, every engine: all
+    equal the oracle's filtered results."""
+    size = 3_000_000
+    ref = M.synth_reference(66, size)
+    b = M.synth_batch(M.make_spec(67, 2400, choices=[300, 520, 777, 900, 1000], err_range=(0.01, 0.08),
+                                  sub_frac=0.7, poisson=11, cap=40, distance=0.1), ref, size)
+    rnd = b["cands"].copy()
+    first = np.r_[0, np.cumsum(np.bincount(rnd["query"], minlength=len(b["qinfo"])))[:-1]]
+    rng = np.random.default_rng(68)
+    rnd["position"][first] = rng.integers(0, size - 1300, len(first))     # no true locus at all
+    true_all = b["cands"].copy()
+    true_all["position"] = true_all["position"][first[true_all["query"]]]  # every candidate at the locus
+    mixed = b["cands"].copy()
+    mixed["position"][::53] = size - 7                                      # skipped: past the end
+    with M.Myers(ref, size, avg_query=800, cands_per_query=11, buffer_bytes=256 << 20) as m:
+        for name, cands in (("random", rnd), ("true-all", true_all), ("mixed", mixed)):
+            bb = dict(b, cands=cands)
+            want = expected(ref, size, bb, 0.1, mode, 0.2)
+            m.set_threshold(0.2)
+            for engine in (M.ENGINE_PLAIN, M.ENGINE_PERSISTENT, M.ENGINE_BINNED):
+                announce(f"threshold probe order {name} mode={mode} engine={engine}")
+                got = m.verify(bb["peq"], bb["qinfo"], bb["cands"], 0.1, flags=mode | engine)
+                np.testing.assert_array_equal(got, want, err_msg=f"{name} engine={engine}")
+        m.set_threshold(-1.0)

@@ -361,9 +361,13 @@ def measure_serving(m, ref, ref_size, loads, seconds, batch_cells=0, engine=0, s
     if no_long_launch and not resident:
         out["no_long_launch"] = True
     with M.Server(m, distance=DISTANCE, flags=flags, max_batch_cells=batch_cells) as sv:
-        # saturated throughput: every pool request arrives at once
-        made = [M.Server.request(*r) for r in pool]
-        lat, wall = sv.run_open_loop([x[0] for x in made], np.zeros(len(made), dtype=np.uint64))
+        # saturated throughput: every pool request arrives at once (best of 3, as the bulk rate)
+        walls = []
+        for _ in range(3):
+            made = [M.Server.request(*r) for r in pool]
+            lat, wall = sv.run_open_loop([x[0] for x in made], np.zeros(len(made), dtype=np.uint64))
+            walls.append(wall)
+        wall = min(walls)
         sat_cps = sum(cands) / (wall / 1e3)
         sat_rps = len(pool) / (wall / 1e3)
         out["bulk_candidates_per_s"] = bulk_cps

@@ -241,7 +241,10 @@ typedef struct {
     uint64_t maxBatchCells;     /* cap per batch on the sum of m * n over its candidates, so
                                    that batches stay short and several run at once
                                    (0 = 5e11; UINT64_MAX = no cap); a request larger than
-                                   the cap runs as a batch of its own */
+                                   the cap runs as a batch of its own.  While more than three
+                                   capped batches' worth of requests wait, a batch may take
+                                   up to four times the cap (draining a backlog at the
+                                   throughput of large batches) */
     uint64_t arenaBytes;        /* MYERS_SERVE_RESIDENT: pinned host memory for admitted
                                    requests' inputs and results (0 = 1 GiB) */
 } myersServeConfig_t;
@@ -272,6 +275,7 @@ typedef struct {
     uint32_t residentLdsBytes;     /* LDS per workgroup */
     uint64_t residentItems;        /* work items published */
     uint64_t liveRequests;         /* submitted and not yet released by a wait */
+    uint64_t drainBatches;         /* batches of several requests above maxBatchCells (backlog drain) */
 } myersServeStats_t;
 
 void myersServeDefaultConfig(myersServeConfig_t *cfg);

@@ -121,6 +121,7 @@ typedef struct {
     slot_t            *slots;
     uint32_t           nslots;
     req_t             *head, *tail;
+    uint64_t           queued_cells;      /* cells of the requests in the queue */
     /* Requests by ticket: table[t - table_base] for table_base <= t <
      * next_ticket, NULL once freed.  A request is freed when its waiter has
      * released it and the server thread no longer refers to it; the table
@@ -166,6 +167,10 @@ static void req_try_free(server_t *sv, req_t *q)
  * candidates (one chain per wave each) to keep the chip busy with three in
  * flight: on the serving mix at 0.7 of the bulk rate, p99 19 ms against 28
  * ms with 2.5e11-cell batches (3 buffers, same box, two runs each). */
+/* Backlog drain: more than DRAIN_BACKLOG capped batches queued -> batches of
+ * up to DRAIN_FACTOR times the cap. */
+#define DRAIN_BACKLOG 3u
+#define DRAIN_FACTOR  4u
 #define DEFAULT_BATCH_CELLS 500000000000ull
 
 /* Sum of m * n over the request's candidates (nominal windows). */
@@ -248,6 +253,18 @@ static int launch_batch(server_t *sv, slot_t *s)
     s->bytes = 0;
     const uint32_t cap_c = (sv->cfg.maxBatchCandidates && sv->cfg.maxBatchCandidates < s->max_cands)
                                ? sv->cfg.maxBatchCandidates : s->max_cands;
+    /* A backlog of more than DRAIN_BACKLOG capped batches is drained in
+     * batches of up to DRAIN_FACTOR times the cap: large batches run their
+     * long reads at the throughput of one queue launch per kernel class
+     * (what a batch of the whole backlog does), where capped batches take
+     * the long-read launch built for latency. */
+    uint64_t cap_cells = sv->cfg.maxBatchCells;
+    int drain = 0;
+    if (cap_cells <= UINT64_MAX / (DRAIN_FACTOR > DRAIN_BACKLOG ? DRAIN_FACTOR : DRAIN_BACKLOG) &&
+        sv->queued_cells > DRAIN_BACKLOG * cap_cells) {
+        cap_cells *= DRAIN_FACTOR;
+        drain = 1;
+    }
     while (sv->head) {
         req_t *q = sv->head;
         const myersRequest_t *r = &q->r;
@@ -255,6 +272,7 @@ static int launch_batch(server_t *sv, slot_t *s)
             /* can never fit a slot: fail it */
             sv->head = q->next;
             if (!sv->head) sv->tail = NULL;
+            sv->queued_cells -= q->cells;
             q->error = MYERS_E_CAPACITY;
             q->t_done_ns = now_ns();
             q->t_admit_ns = q->t_done_ns;
@@ -265,7 +283,7 @@ static int launch_batch(server_t *sv, slot_t *s)
         if (ne + r->numPEQEntries > s->max_entries || nq + r->numQueries > s->max_queries ||
             nc + r->numCandidates > cap_c)
             break;
-        if (s->nreq > 0 && cells + q->cells > sv->cfg.maxBatchCells) break;
+        if (s->nreq > 0 && cells + q->cells > cap_cells) break;
         if (sv->inflight_bytes + q->bytes > sv->cfg.hbmBudgetBytes && sv->inflight_bytes > 0) break;
         if (s->nreq == s->cap_req) {
             uint32_t cap = s->cap_req ? 2u * s->cap_req : 64u;
@@ -282,6 +300,7 @@ static int launch_batch(server_t *sv, slot_t *s)
         }
         sv->head = q->next;
         if (!sv->head) sv->tail = NULL;
+        sv->queued_cells -= q->cells;
         memcpy(peq + ne, r->peq, (size_t)r->numPEQEntries * sizeof(qryEntry_t));
         for (uint32_t i = 0; i < r->numQueries; i++) {
             qinfo[nq + i].posEntry = r->qinfo[i].posEntry + ne;
@@ -319,6 +338,7 @@ static int launch_batch(server_t *sv, slot_t *s)
     }
     s->busy = 1;
     sv->stats.batches++;
+    if (drain && s->nreq > 1 && cells > sv->cfg.maxBatchCells) sv->stats.drainBatches++;
     sv->stats.admitMs += (double)(now_ns() - t_start) / 1e6;
     if (s->nreq > sv->stats.maxBatchRequests) sv->stats.maxBatchRequests = s->nreq;
     return 1;
@@ -634,6 +654,7 @@ static void *resident_main(void *arg)
             if (!res_admit(sv, q)) break;
             sv->head = q->next;
             if (!sv->head) sv->tail = NULL;
+            sv->queued_cells -= q->cells;
             admitted = 1;
         }
         if (admitted) {
@@ -790,6 +811,7 @@ int myersServeSubmit(void *server, const myersRequest_t *req, uint64_t *ticket)
     if (sv->tail) sv->tail->next = q;
     else sv->head = q;
     sv->tail = q;
+    sv->queued_cells += q->cells;
     *ticket = q->ticket;
     pthread_cond_signal(&sv->cv_work);
     pthread_mutex_unlock(&sv->mu);

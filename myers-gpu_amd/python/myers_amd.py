@@ -92,7 +92,8 @@ class ServeStats(ctypes.Structure):
                 ("maxBatchRequests", ctypes.c_uint64), ("busyMs", ctypes.c_double),
                 ("admitMs", ctypes.c_double), ("completeMs", ctypes.c_double),
                 ("residentWorkgroups", ctypes.c_uint32), ("residentLdsBytes", ctypes.c_uint32),
-                ("residentItems", ctypes.c_uint64), ("liveRequests", ctypes.c_uint64)]
+                ("residentItems", ctypes.c_uint64), ("liveRequests", ctypes.c_uint64),
+                ("drainBatches", ctypes.c_uint64)]
 
 
 class SynthSpec(ctypes.Structure):

@@ -243,3 +243,28 @@ def test_server_releases_requests(serve):
             assert peak <= 16
             with pytest.raises(M.MyersError):
                 sv.wait(tickets[-1])
+
+
+def test_backlog_drain_batches():
+    """A backlog of more than three capped batches is drained in batches of
+    up to four times the cap (which then take the throughput path for their
+    long reads); every request's results stay those it gets alone."""
+    announce("serve backlog drain")
+    ref, reqs = make_pool(47)
+    cells = sum(int((r[1][r[2]["query"], 1].astype(np.int64) *
+                     np.floor(r[1][r[2]["query"], 1].astype(np.float32) * np.float32(1.2))).sum())
+                for r in reqs)
+    cap = max(1, cells // 40)
+    with M.Myers(ref, REF_SIZE, num_buffers=3, avg_query=800, cands_per_query=11, buffer_bytes=256 << 20) as m:
+        alone = [m.verify(*r, 0.1) for r in reqs]
+        with M.Server(m, distance=0.1, max_batch_cells=cap) as sv:
+            made = [M.Server.request(*r) for r in reqs]
+            tickets = [sv.submit(x[0]) for x in made]
+            for t in tickets:
+                assert sv.wait(t) >= 0.0
+            st = sv.stats()
+            assert st.requests == len(reqs)
+            assert st.drainBatches >= 1, (st.batches, cap, cells)
+        for (req, keep), a, r in zip(made, alone, reqs):
+            np.testing.assert_array_equal(keep[3], a)
+            np.testing.assert_array_equal(a, oracle_req(ref, r))

@@ -1137,8 +1137,8 @@ __global__ __launch_bounds__(64 * Q) void verify_wide_persistent_kernel(myers_ve
     X(1, 12) X(1, 13) X(1, 14) X(1, 16) X(1, 19) X(1, 20) X(1, 22) X(1, 24) X(1, 25) X(1, 28)    \
     X(1, 29) X(1, 32)                                                                            \
     X(2, 8) X(2, 12) X(2, 16) X(4, 8) X(4, 12) X(4, 16) X(8, 8) X(8, 12) X(8, 16)                \
-    X(16, 8) X(16, 10) X(16, 12) X(16, 16) X(32, 7) X(32, 8) X(32, 10) X(32, 12) X(32, 16)        \
-    X(64, 7) X(64, 8) X(64, 10) X(64, 12)
+    X(16, 8) X(16, 9) X(16, 10) X(16, 11) X(16, 12) X(16, 16) X(32, 7) X(32, 8) X(32, 9)          \
+    X(32, 10) X(32, 11) X(32, 12) X(32, 16) X(64, 7) X(64, 8) X(64, 9) X(64, 10) X(64, 12)
 
 // Multi-wave variants (lanes = 64 Q per candidate, one candidate per
 // workgroup of Q waves): long reads with lower per-candidate latency.

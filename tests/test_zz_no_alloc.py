@@ -3,7 +3,8 @@ myersVerifyDevice (its pipeline workspace is the library's static device
 array).  Called through the C-ABI and compared bit-exactly with
 myersVerifyDevice on the same device batches -- two calls in a row, the
 second smaller, so the second runs over what the first left in the shared
-array -- for every engine and both window modes; the oracle checks both."""
+array -- for every engine and both window modes, and in threshold mode; the
+oracle checks both."""
 import ctypes
 
 import numpy as np
@@ -56,6 +57,16 @@ def test_no_alloc_matches_verify_device(mode):
                 np.testing.assert_array_equal(got, want)
                 exp = O.myers_batch(ref, REF_SIZE, b["peq"], b["qinfo"], b["cands"], 0.1, mode)
                 np.testing.assert_array_equal(got, exp)
+        # threshold mode: the band probe and the span order run in the same pipeline
+        m.set_threshold(0.2)
+        for name, b in (("first", big), ("second, smaller", small)):
+            announce(f"no-alloc threshold mode={mode} {name}: {len(b['cands'])} candidates")
+            want = run(m, L.myersVerifyDevice, b, mode | M.ENGINE_PERSISTENT)
+            got = run(m, L.myersVerifyDeviceNoAlloc, b, mode | M.ENGINE_PERSISTENT)
+            np.testing.assert_array_equal(got, want)
+            full = O.myers_batch(ref, REF_SIZE, b["peq"], b["qinfo"], b["cands"], 0.1, mode)
+            np.testing.assert_array_equal(got, O.apply_threshold(b["qinfo"], b["cands"], full, 0.2))
+        m.set_threshold(-1.0)
 
 
 def test_no_alloc_refuses_oversize_and_graphs():

@@ -77,10 +77,10 @@ CONFIGS = {
 }
 DISTANCE = 0.1
 # HBM bytes per launch of the dominant kernel under the auto engine (the queue
-# launch for both), from FETCH_SIZE and WRITE_SIZE counter runs scaled by a 1 GiB
+# launch for each), from FETCH_SIZE and WRITE_SIZE counter runs scaled by a 1 GiB
 # copy calibration (profiles/verify_queue_kernel_config2.md,
-# profiles/hbm_traffic_verify_queue_config3.md).
-MEASURED_TRAFFIC = {(2, "aligned"): 1.624e9, (3, "aligned"): 9.547e9}
+# profiles/hbm_traffic_verify_queue_config3.md, profiles/hbm_traffic_verify_queue_config4.md).
+MEASURED_TRAFFIC = {(2, "aligned"): 1.624e9, (3, "aligned"): 9.547e9, (4, "aligned"): 1.536e9}
 
 
 def log(msg):
